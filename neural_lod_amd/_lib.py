@@ -1,0 +1,97 @@
+"""ctypes binding of libnlod.so (the C ABI declared in include/nlod.h).
+
+The library must have been built (``python -m neural_lod_amd.build`` or
+``__graft_entry__.build()``); there is no fallback implementation.  torch is
+imported first so that libnlod shares torch's HIP runtime instance.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+
+import torch  # noqa: F401  (loads the HIP runtime libnlod links against)
+
+LIB_PATH = Path(__file__).resolve().parent / "libnlod.so"
+
+_lock = threading.Lock()
+_lib: C.CDLL | None = None
+
+NLOD_OK = 0
+STATUS_NAMES = {
+    1: "NLOD_ERR_INVALID_ARGUMENT",
+    2: "NLOD_ERR_CONFIG",
+    3: "NLOD_ERR_UNSUPPORTED",
+    4: "NLOD_ERR_HIP",
+    5: "NLOD_ERR_OUT_OF_MEMORY",
+    6: "NLOD_ERR_STATE",
+}
+
+
+class NlodError(RuntimeError):
+    """A libnlod call returned a non-zero status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class VoxInfo(C.Structure):
+    _fields_ = [
+        ("bbox_min", C.c_float * 3),
+        ("bbox_max", C.c_float * 3),
+        ("gridsize", C.c_uint32 * 3),
+        ("reserved0", C.c_uint32),
+        ("n_triangles", C.c_uint64),
+        ("unit", C.c_float * 3),
+        ("reserved1", C.c_uint32),
+    ]
+
+
+_vp = C.c_void_p
+_fp = C.POINTER(C.c_float)
+_u32 = C.c_uint32
+_u64 = C.c_uint64
+_i32 = C.c_int
+
+# name -> argtypes (every function returns int status unless listed in _RESTYPES)
+SIGNATURES: dict[str, list] = {
+    "nlod_last_error": [],
+    "nlod_version": [],
+    "nlod_vox_table_words": [_u32, C.POINTER(_u64)],
+    "nlod_voxinfo_from_bbox": [_fp, _fp, _u32, _u32, _u64, C.POINTER(VoxInfo)],
+    "nlod_mesh_bbox": [_vp, _vp, _u64, _fp, _fp],
+    "nlod_voxelize": [_vp, C.POINTER(VoxInfo), _vp, _vp, _u32],
+    "nlod_voxelize_mesh": [_vp, _vp, _u64, _u32, _u32, _vp, C.POINTER(VoxInfo)],
+}
+_RESTYPES = {"nlod_last_error": C.c_char_p, "nlod_version": C.c_char_p}
+
+
+def lib() -> C.CDLL:
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not LIB_PATH.exists():
+                raise RuntimeError(
+                    f"libnlod.so not found at {LIB_PATH}: build it first "
+                    "(python -m neural_lod_amd.build); there is no CPU fallback")
+            L = C.CDLL(str(LIB_PATH))
+            for name, args in SIGNATURES.items():
+                fn = getattr(L, name)
+                fn.argtypes = args
+                fn.restype = _RESTYPES.get(name, C.c_int)
+            _lib = L
+    return _lib
+
+
+def call(name: str, *args) -> None:
+    """Call a status-returning libnlod function; raise NlodError on failure."""
+    L = lib()
+    status = getattr(L, name)(*args)
+    if status != NLOD_OK:
+        msg = L.nlod_last_error().decode(errors="replace")
+        raise NlodError(status, msg)
+
+
+def exported_symbols() -> list[str]:
+    return list(SIGNATURES)

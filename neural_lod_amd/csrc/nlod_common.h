@@ -1,0 +1,67 @@
+// Shared host-side plumbing of libnlod: error capture for the C ABI and HIP
+// call checking.  Device helpers live next to the kernels that use them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "nlod.h"
+
+namespace nlod {
+
+// Error thrown inside the library and turned into a status code at the ABI.
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string& msg) : std::runtime_error(msg), status(s) {}
+};
+
+void set_last_error(const std::string& msg);
+void clear_last_error();
+
+[[noreturn]] inline void fail(int status, const std::string& msg) { throw Error(status, msg); }
+
+inline void check_hip(hipError_t e, const char* what, const char* file, int line) {
+    if (e != hipSuccess) {
+        fail(e == hipErrorOutOfMemory ? NLOD_ERR_OUT_OF_MEMORY : NLOD_ERR_HIP,
+             std::string(what) + " failed: " + hipGetErrorString(e) + " (" + file + ":" +
+                 std::to_string(line) + ")");
+    }
+}
+
+#define NLOD_HIP(call) ::nlod::check_hip((call), #call, __FILE__, __LINE__)
+#define NLOD_CHECK(cond, msg)                                                     \
+    do {                                                                          \
+        if (!(cond)) ::nlod::fail(NLOD_ERR_INVALID_ARGUMENT, std::string(msg));  \
+    } while (0)
+#define NLOD_LAUNCH_CHECK() NLOD_HIP(hipGetLastError())
+
+// Runs f() and converts any exception into an ABI status.
+template <typename F>
+int guarded(F&& f) {
+    try {
+        clear_last_error();
+        f();
+        return NLOD_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return NLOD_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return NLOD_ERR_INVALID_ARGUMENT;
+    } catch (...) {
+        set_last_error("unknown error");
+        return NLOD_ERR_INVALID_ARGUMENT;
+    }
+}
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline uint32_t div_round_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace nlod

@@ -1,0 +1,100 @@
+"""ORACLE — test infrastructure only.
+
+Python access to the CPU oracle (``liboracle.so``, built from the C sources in
+this directory by ``make -C oracle``).  Only tests/, ``__graft_entry__.smoke()``
+and bench.py's CPU-baseline leg import this module, and only as the checker.
+The product (``neural_lod_amd``) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_lib = None
+
+
+class VoxInfo(C.Structure):
+    _fields_ = [
+        ("bbox_min", C.c_float * 3),
+        ("bbox_max", C.c_float * 3),
+        ("gridsize", C.c_uint32 * 3),
+        ("pad0", C.c_uint32),
+        ("n_triangles", C.c_uint64),
+        ("unit", C.c_float * 3),
+        ("pad1", C.c_uint32),
+    ]
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE), "-B"], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        fp = C.POINTER(C.c_float)
+        u32p = C.POINTER(C.c_uint32)
+        L.nlo_mesh_bbox.argtypes = [fp, C.c_uint64, fp, fp]
+        L.nlo_voxinfo_from_bbox.argtypes = [fp, fp, C.c_uint32, C.c_float, C.c_uint64, C.POINTER(VoxInfo)]
+        L.nlo_voxelize_surface.argtypes = [C.POINTER(VoxInfo), fp, u32p, C.c_int]
+        L.nlo_voxelize_solid.argtypes = [C.POINTER(VoxInfo), fp, u32p, C.c_int]
+        L.nlo_omp_max_threads.restype = C.c_int
+        L.nlo_omp_set_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def mesh_bbox(tris: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    tris = np.ascontiguousarray(tris, dtype=np.float32)
+    mn = np.zeros(3, np.float32)
+    mx = np.zeros(3, np.float32)
+    lib().nlo_mesh_bbox(_fptr(tris), tris.size // 9, _fptr(mn), _fptr(mx))
+    return mn, mx
+
+
+def voxinfo(bbox_min, bbox_max, res: int, pad: bool, n_tri: int) -> VoxInfo:
+    mn = np.ascontiguousarray(bbox_min, dtype=np.float32)
+    mx = np.ascontiguousarray(bbox_max, dtype=np.float32)
+    info = VoxInfo()
+    lib().nlo_voxinfo_from_bbox(_fptr(mn), _fptr(mx), res, 1e-4 if pad else 0.0, n_tri, C.byref(info))
+    return info
+
+
+def table_words(res: int) -> int:
+    return (res ** 3 + 31) // 32
+
+
+def voxelize(tris: np.ndarray, res: int, solid: bool = False, pad: bool = False, morton: bool = False,
+             info: VoxInfo | None = None, threads: int | None = None) -> tuple[np.ndarray, VoxInfo]:
+    """Bit table (uint32[ceil(res^3/32)]) of the reference CPU voxelizer."""
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    if info is None:
+        mn, mx = mesh_bbox(tris)
+        info = voxinfo(mn, mx, res, pad, tris.shape[0])
+    table = np.zeros(table_words(res), np.uint32)
+    L = lib()
+    if threads is not None:
+        L.nlo_omp_set_threads(int(threads))
+    fn = L.nlo_voxelize_solid if solid else L.nlo_voxelize_surface
+    fn(C.byref(info), _fptr(tris), table.ctypes.data_as(C.POINTER(C.c_uint32)), int(morton))
+    return table, info
+
+
+def unpack_bits(table: np.ndarray, res: int) -> np.ndarray:
+    """Occupancy as bool[z][y][x] from the MSB-first bit table."""
+    bits = np.unpackbits(table.astype(">u4").view(np.uint8))[: res ** 3]
+    return bits.reshape(res, res, res).astype(bool)
