@@ -97,6 +97,86 @@ int nlod_voxelize(void* stream, const nlod_voxinfo* info, const float* d_triangl
 int nlod_voxelize_mesh(void* stream, const float* d_triangles, uint64_t n_triangles, uint32_t res,
                        uint32_t flags, uint32_t* d_table, nlod_voxinfo* info_out);
 
+/* ----------------------------------------- network with input encoding */
+
+typedef struct nlod_model nlod_model;
+typedef struct nlod_ctx nlod_ctx;
+
+enum nlod_precision {
+    NLOD_PRECISION_FP32 = 0, /* plain form: FP32 throughout */
+    NLOD_PRECISION_FP16 = 1  /* FP16 MFMA GEMMs with FP32 accumulation; FP32 grid interpolation */
+};
+
+/* EGradientMode (tiny-cuda-nn include/tiny-cuda-nn/object.h:115-119) */
+enum nlod_grad_mode { NLOD_GRAD_IGNORE = 0, NLOD_GRAD_OVERWRITE = 1, NLOD_GRAD_ACCUMULATE = 2 };
+
+typedef struct nlod_model_info {
+    uint64_t n_params;         /* network + encoding parameters (float) */
+    uint64_t n_network_params; /* leading part of the parameter vector */
+    uint32_t input_width;      /* floats per input row */
+    uint32_t encoded_width;    /* padded encoding width = network input width */
+    uint32_t padded_output_width;
+    uint32_t output_width;
+    uint32_t network_width;
+    uint32_t n_hidden_layers;
+    uint32_t precision;
+    uint32_t reserved;
+} nlod_model_info;
+
+/* Model from tcnn JSON (encoding + network).  Replaces
+ * tcnn::NetworkWithInputEncoding<T>(n_dims_to_encode, n_output_dims, encoding, network)
+ * (tiny-cuda-nn include/tiny-cuda-nn/network_with_input_encoding.h:53-54),
+ * as built by rptr/cuda/neural-lod/neural_lod_learning_init.cu:663-664,898-899.
+ * Encodings: Composite, RepeatedComposite, HashGrid/Grid/DenseGrid/TiledGrid,
+ * SphericalHarmonics, Identity.  Network: FullyFusedMLP (widths 16-128, no
+ * biases), hidden ReLU/None, output None/ReLU/Sigmoid/Exponential. */
+int nlod_model_create(uint32_t n_input_dims, uint32_t n_output_dims, const char* encoding_json,
+                      const char* network_json, int precision, nlod_model** model_out);
+int nlod_model_destroy(nlod_model* model);
+
+/* n_params / widths (object.h:362-367, network_with_input_encoding.h:130-165). */
+int nlod_model_get_info(const nlod_model* model, nlod_model_info* info_out);
+
+/* JSON description of the flattened layout (segments, offsets, level tables).
+ * Writes at most buf_len bytes (NUL-terminated); *needed gets the full size. */
+int nlod_model_describe(const nlod_model* model, char* buf, size_t buf_len, size_t* needed);
+
+/* tcnn Trainer initialisation (trainer.h:53-92: pcg32 from seed_seq{seed},
+ * Xavier-uniform matrices, grid U[-1e-4,1e-4]) into a host float[n_params]. */
+int nlod_model_init_params(const nlod_model* model, uint32_t seed, float* params_host);
+
+/* Binds device parameter vectors (float[n_params], caller-owned, must stay
+ * valid): training params and inference params (NULL = same).  Replaces
+ * set_params(params, inference_params, gradients) (object.h:73-79).  For the
+ * FP16 form this also repacks the FP16 MFMA weight copies on `stream`; call it
+ * again after the parameters change. */
+int nlod_model_set_params(nlod_model* model, void* stream, const float* d_params, const float* d_inference_params);
+
+/* Inference: d_in float[n][input_width] -> d_out float[n][output_width].  Any
+ * n (no multiple-of-128 restriction).  Replaces DifferentiableObject::inference
+ * (object.h:147-169) as called from wavefront_neural_throughput_visibility_lod.cu:595,662. */
+int nlod_model_inference(nlod_model* model, void* stream, uint32_t n, const float* d_in, float* d_out);
+
+/* Forward with saved context (object.h:182-203); d_out may be NULL. */
+int nlod_model_forward(nlod_model* model, void* stream, uint32_t n, const float* d_in, float* d_out,
+                       int use_inference_params, int prepare_input_gradients, nlod_ctx** ctx_out);
+
+/* Backward (object.h:223-259): d_dL_dout float[n][output_width] ->
+ * d_dL_dparams float[n_params] (per grad_mode) and, if non-NULL,
+ * d_dL_dinput float[n][input_width] (SH/identity encodings only). */
+int nlod_model_backward(nlod_model* model, void* stream, nlod_ctx* ctx, const float* d_in, const float* d_dL_dout,
+                        float* d_dL_dparams, float* d_dL_dinput, int grad_mode);
+int nlod_ctx_destroy(nlod_ctx* ctx);
+
+/* Encoding alone (tcnn Encoding::inference, encoding.h:59-88): encoded
+ * features as SoA float[encoded_width][n] (row r of query i at r*n + i). */
+int nlod_model_encode(nlod_model* model, void* stream, uint32_t n, const float* d_in, float* d_encoded,
+                      int use_inference_params);
+/* Grid parameter gradients from dL/d(encoded) SoA float[encoded_width][n]
+ * (GridEncoding::backward_impl, grid.h:1102-1193). */
+int nlod_model_encode_backward(nlod_model* model, void* stream, uint32_t n, const float* d_in,
+                               const float* d_dL_dencoded, float* d_dL_dparams, int grad_mode);
+
 #ifdef __cplusplus
 }
 #endif

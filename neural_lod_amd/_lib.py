@@ -63,7 +63,34 @@ SIGNATURES: dict[str, list] = {
     "nlod_mesh_bbox": [_vp, _vp, _u64, _fp, _fp],
     "nlod_voxelize": [_vp, C.POINTER(VoxInfo), _vp, _vp, _u32],
     "nlod_voxelize_mesh": [_vp, _vp, _u64, _u32, _u32, _vp, C.POINTER(VoxInfo)],
+    "nlod_model_create": [_u32, _u32, C.c_char_p, C.c_char_p, _i32, C.POINTER(_vp)],
+    "nlod_model_destroy": [_vp],
+    "nlod_model_get_info": [_vp, _vp],
+    "nlod_model_describe": [_vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)],
+    "nlod_model_init_params": [_vp, _u32, _vp],
+    "nlod_model_set_params": [_vp, _vp, _vp, _vp],
+    "nlod_model_inference": [_vp, _vp, _u32, _vp, _vp],
+    "nlod_model_forward": [_vp, _vp, _u32, _vp, _vp, _i32, _i32, C.POINTER(_vp)],
+    "nlod_model_backward": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32],
+    "nlod_ctx_destroy": [_vp],
+    "nlod_model_encode": [_vp, _vp, _u32, _vp, _vp, _i32],
+    "nlod_model_encode_backward": [_vp, _vp, _u32, _vp, _vp, _vp, _i32],
 }
+
+
+class ModelInfo(C.Structure):
+    _fields_ = [
+        ("n_params", C.c_uint64),
+        ("n_network_params", C.c_uint64),
+        ("input_width", C.c_uint32),
+        ("encoded_width", C.c_uint32),
+        ("padded_output_width", C.c_uint32),
+        ("output_width", C.c_uint32),
+        ("network_width", C.c_uint32),
+        ("n_hidden_layers", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
 _RESTYPES = {"nlod_last_error": C.c_char_p, "nlod_version": C.c_char_p}
 
 

@@ -1,0 +1,279 @@
+// NetworkWithInputEncoding (tiny-cuda-nn include/tiny-cuda-nn/network_with_input_encoding.h)
+// and its C ABI: parameters are bound by pointer (network first, then the
+// encoding, network_with_input_encoding.h:113-128); inference, forward with a
+// saved context, and backward producing parameter / input gradients.
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+
+#include "model.h"
+#include "rng.h"
+
+namespace nlod {
+
+Model::Model(uint32_t n_in, uint32_t n_out, const Json& enc, const Json& net, int prec) : precision(prec) {
+    if (prec != PREC_FP32 && prec != PREC_FP16) fail(NLOD_ERR_INVALID_ARGUMENT, "precision must be FP32 (0) or FP16 (1)");
+    L = build_layout(n_in, n_out, enc, net);
+    if (precision == PREC_FP16) {
+        if (!mfma_supported(L)) fail(NLOD_ERR_UNSUPPORTED, "this layout is not supported by the FP16 MFMA kernels");
+        program.ensure(mfma_program_bytes());
+        mfma_upload_program(L, program.p);
+    }
+}
+
+void Model::set_params(hipStream_t s, const float* p, const float* p_inf) {
+    NLOD_CHECK(p != nullptr, "params pointer is null");
+    params = p;
+    params_inference = p_inf ? p_inf : p;
+    if (precision == PREC_FP16) {
+        const size_t bytes = mfma_packed_bytes(L.net);
+        packed.ensure(bytes);
+        packed_inference.ensure(bytes);
+        mfma_pack_weights(s, L.net, params, packed.p);
+        mfma_pack_weights(s, L.net, params_inference, packed_inference.p);
+    }
+}
+
+void Model::require_params(bool inference) const {
+    if (!(inference ? params_inference : params))
+        fail(NLOD_ERR_STATE, "parameters are not set (nlod_model_set_params)");
+}
+
+void Model::encode(hipStream_t s, uint32_t N, const float* in, float* enc_soa, bool inference) {
+    require_params(inference);
+    encode_forward_f32(s, L, weights(inference), in, N, enc_soa);
+}
+
+void Model::inference(hipStream_t s, uint32_t N, const float* in, float* out) {
+    require_params(true);
+    if (N == 0) return;
+    if (precision == PREC_FP16) {
+        mfma_fused_inference(s, L, program.p, params_inference, packed_inference.p, in, N, out);
+        return;
+    }
+    ws_enc.ensure(sizeof(float) * (size_t)L.enc_width * N);
+    encode_forward_f32(s, L, params_inference, in, N, ws_enc.as<float>());
+    mlp_forward_f32(s, L.net, params_inference, ws_enc.as<float>(), N, out, nullptr);
+}
+
+std::unique_ptr<ForwardCtx> Model::forward(hipStream_t s, uint32_t N, const float* in, float* out, bool use_inference,
+                                           bool prepare_input_gradients) {
+    require_params(use_inference);
+    if (precision == PREC_FP16)
+        fail(NLOD_ERR_UNSUPPORTED, "FP16 forward with saved context is not available in this build");
+    auto ctx = std::make_unique<ForwardCtx>();
+    ctx->N = N;
+    ctx->use_inference_params = use_inference;
+    ctx->prepare_input_gradients = prepare_input_gradients;
+    if (N == 0) return ctx;
+    const NetShape& n = L.net;
+    ctx->enc.ensure(sizeof(float) * (size_t)L.enc_width * N);
+    ctx->hidden.ensure(sizeof(float) * (size_t)n.n_hidden_layers * n.width * N);
+    ctx->out.ensure(sizeof(float) * (size_t)n.n_output * N);
+    const float* w = weights(use_inference);
+    encode_forward_f32(s, L, w, in, N, ctx->enc.as<float>());
+    mlp_forward_f32(s, n, w, ctx->enc.as<float>(), N, ctx->out.as<float>(), ctx->hidden.as<float>());
+    if (out)
+        NLOD_HIP(hipMemcpyAsync(out, ctx->out.p, sizeof(float) * (size_t)n.n_output * N, hipMemcpyDeviceToDevice, s));
+    return ctx;
+}
+
+void Model::backward(hipStream_t s, ForwardCtx& ctx, const float* in, const float* d_out, float* d_params, float* d_in,
+                     int grad_mode) {
+    NLOD_CHECK(grad_mode >= GRAD_IGNORE && grad_mode <= GRAD_ACCUMULATE, "invalid gradient mode");
+    NLOD_CHECK(grad_mode == GRAD_IGNORE || d_params, "parameter-gradient pointer is null");
+    if (d_in && !encode_backward_input_supported(L))
+        fail(NLOD_ERR_UNSUPPORTED, "input gradients through a grid encoding are not supported");
+    const uint32_t N = ctx.N;
+    const NetShape& n = L.net;
+    if (N == 0) {
+        if (grad_mode == GRAD_OVERWRITE) fill_f32(s, d_params, L.n_params, 0.0f);
+        return;
+    }
+    NLOD_CHECK(in && d_out, "input / output-gradient pointer is null");
+    const float* w = weights(ctx.use_inference_params);
+    ws_delta.ensure(sizeof(float) * (size_t)n.n_hidden_layers * n.width * N);
+    ws_delta_out.ensure(sizeof(float) * (size_t)n.n_output_padded * N);
+    const bool need_denc = (grad_mode != GRAD_IGNORE && L.n_params > L.n_net_params) || d_in;
+    if (need_denc) ws_denc.ensure(sizeof(float) * (size_t)L.enc_width * N);
+    mlp_backward_delta_f32(s, n, w, ctx.hidden.as<float>(), ctx.out.as<float>(), d_out, N, ws_delta.as<float>(),
+                           ws_delta_out.as<float>(), need_denc ? ws_denc.as<float>() : nullptr);
+    if (grad_mode != GRAD_IGNORE) {
+        const size_t wsf = mlp_weight_grad_workspace_floats(n, N);
+        ws_wgrad.ensure(sizeof(float) * wsf);
+        mlp_weight_grad_f32(s, n, ctx.enc.as<float>(), ctx.hidden.as<float>(), ws_delta.as<float>(),
+                            ws_delta_out.as<float>(), N, d_params, ws_wgrad.as<float>(), wsf,
+                            grad_mode == GRAD_ACCUMULATE);
+        if (L.n_params > L.n_net_params) {
+            if (grad_mode == GRAD_OVERWRITE) fill_f32(s, d_params + L.n_net_params, L.n_params - L.n_net_params, 0.0f);
+            encode_backward_params(s, L, in, N, ws_denc.as<float>(), d_params);
+        }
+    }
+    if (d_in) encode_backward_input(s, L, in, N, ws_denc.as<float>(), d_in);
+}
+
+// Parameter initialisation of tiny-cuda-nn's Trainer (trainer.h:53-92):
+// pcg32 seeded from std::seed_seq{seed}; network matrices Xavier-uniform on the
+// host in order (gpu_matrix.h:291-306), then each grid table U[-1e-4, 1e-4]
+// drawn as generate_random_uniform does it (random.h:63-93: thread t of
+// ceil(n/4) threads takes stream values 4t..4t+3 for elements t + j*ceil(n/4)).
+void Model::init_params_host(uint32_t seed, float* out) const {
+    std::seed_seq seq{seed};
+    std::vector<uint32_t> seeds(2);
+    seq.generate(seeds.begin(), seeds.end());
+    Pcg32 rng(seeds.front());
+    const NetShape& n = L.net;
+    float* p = out;
+    auto xavier = [&](uint32_t rows, uint32_t cols) {
+        const float scale = std::sqrt(6.0f / (float)(cols + rows));
+        for (size_t k = 0; k < (size_t)rows * cols; ++k) p[k] = rng.next_float() * 2.0f * scale - scale;
+        p += (size_t)rows * cols;
+    };
+    xavier(n.width, n.n_input);
+    for (uint32_t l = 1; l < n.n_hidden_layers; ++l) xavier(n.width, n.width);
+    xavier(n.n_output_padded, n.width);
+    for (const Segment& s : L.segs) {
+        if (s.type != SEG_GRID || !s.first_use) continue;
+        const size_t count = s.n_params;
+        const size_t n_threads = (count + 3) / 4;
+        const float lo = -1e-4f, hi = 1e-4f, span = hi - lo;
+        Pcg32 r = rng;
+        float* t = out + s.param_offset;
+        for (size_t pos = 0; pos < n_threads * 4; ++pos) {
+            const float v = r.next_float();
+            const size_t idx = pos / 4 + n_threads * (pos % 4);
+            if (idx < count) t[idx] = std::fma(v, span, lo);
+        }
+        rng.advance(count);
+    }
+}
+
+}  // namespace nlod
+
+// ============================================================== C ABI
+struct nlod_model {
+    std::unique_ptr<nlod::Model> m;
+};
+struct nlod_ctx {
+    std::unique_ptr<nlod::ForwardCtx> c;
+    nlod_model* owner = nullptr;
+};
+
+extern "C" {
+
+int nlod_model_create(uint32_t n_input_dims, uint32_t n_output_dims, const char* encoding_json,
+                      const char* network_json, int precision, nlod_model** out) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(out && encoding_json && network_json, "null argument");
+        *out = nullptr;
+        const nlod::Json enc = nlod::Json::parse(encoding_json);
+        const nlod::Json net = nlod::Json::parse(network_json);
+        auto h = std::make_unique<nlod_model>();
+        h->m = std::make_unique<nlod::Model>(n_input_dims, n_output_dims, enc, net, precision);
+        *out = h.release();
+    });
+}
+
+int nlod_model_destroy(nlod_model* model) {
+    return nlod::guarded([&] { delete model; });
+}
+
+int nlod_model_get_info(const nlod_model* model, nlod_model_info* info) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(model && info, "null argument");
+        const nlod::Layout& L = model->m->L;
+        std::memset(info, 0, sizeof(*info));
+        info->n_params = L.n_params;
+        info->n_network_params = L.n_net_params;
+        info->input_width = L.n_input_dims;
+        info->encoded_width = L.enc_width;
+        info->padded_output_width = L.net.n_output_padded;
+        info->output_width = L.net.n_output;
+        info->network_width = L.net.width;
+        info->n_hidden_layers = L.net.n_hidden_layers;
+        info->precision = (uint32_t)model->m->precision;
+    });
+}
+
+int nlod_model_describe(const nlod_model* model, char* buf, size_t buf_len, size_t* needed) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(model, "null model");
+        const std::string d = model->m->L.describe();
+        if (needed) *needed = d.size() + 1;
+        if (buf && buf_len) {
+            const size_t n = std::min(buf_len - 1, d.size());
+            std::memcpy(buf, d.data(), n);
+            buf[n] = '\0';
+        }
+    });
+}
+
+int nlod_model_init_params(const nlod_model* model, uint32_t seed, float* params_host) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(model && params_host, "null argument");
+        model->m->init_params_host(seed, params_host);
+    });
+}
+
+int nlod_model_set_params(nlod_model* model, void* stream, const float* d_params, const float* d_inference_params) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(model, "null model");
+        model->m->set_params(nlod::as_stream(stream), d_params, d_inference_params);
+    });
+}
+
+int nlod_model_encode(nlod_model* model, void* stream, uint32_t n, const float* d_in, float* d_encoded,
+                      int use_inference_params) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(model && (n == 0 || (d_in && d_encoded)), "null argument");
+        model->m->encode(nlod::as_stream(stream), n, d_in, d_encoded, use_inference_params != 0);
+    });
+}
+
+int nlod_model_encode_backward(nlod_model* model, void* stream, uint32_t n, const float* d_in,
+                               const float* d_dL_dencoded, float* d_dL_dparams, int grad_mode) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(model && d_dL_dparams, "null argument");
+        nlod::Model& m = *model->m;
+        NLOD_CHECK(grad_mode == nlod::GRAD_OVERWRITE || grad_mode == nlod::GRAD_ACCUMULATE, "invalid gradient mode");
+        hipStream_t s = nlod::as_stream(stream);
+        if (grad_mode == nlod::GRAD_OVERWRITE) nlod::fill_f32(s, d_dL_dparams, m.L.n_params, 0.0f);
+        if (n) nlod::encode_backward_params(s, m.L, d_in, n, d_dL_dencoded, d_dL_dparams);
+    });
+}
+
+int nlod_model_inference(nlod_model* model, void* stream, uint32_t n, const float* d_in, float* d_out) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(model && (n == 0 || (d_in && d_out)), "null argument");
+        model->m->inference(nlod::as_stream(stream), n, d_in, d_out);
+    });
+}
+
+int nlod_model_forward(nlod_model* model, void* stream, uint32_t n, const float* d_in, float* d_out,
+                       int use_inference_params, int prepare_input_gradients, nlod_ctx** ctx_out) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(model && ctx_out && (n == 0 || d_in), "null argument");
+        *ctx_out = nullptr;
+        auto c = std::make_unique<nlod_ctx>();
+        c->c = model->m->forward(nlod::as_stream(stream), n, d_in, d_out, use_inference_params != 0,
+                                 prepare_input_gradients != 0);
+        c->owner = model;
+        *ctx_out = c.release();
+    });
+}
+
+int nlod_model_backward(nlod_model* model, void* stream, nlod_ctx* ctx, const float* d_in, const float* d_dL_dout,
+                        float* d_dL_dparams, float* d_dL_dinput, int grad_mode) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(model && ctx && ctx->c, "null argument");
+        if (ctx->owner != model) nlod::fail(NLOD_ERR_STATE, "context belongs to another model");
+        model->m->backward(nlod::as_stream(stream), *ctx->c, d_in, d_dL_dout, d_dL_dparams, d_dL_dinput, grad_mode);
+    });
+}
+
+int nlod_ctx_destroy(nlod_ctx* ctx) {
+    return nlod::guarded([&] { delete ctx; });
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// NetworkWithInputEncoding on gfx950: the model object behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "kernels.h"
+#include "layout.h"
+#include "nlod_common.h"
+
+namespace nlod {
+
+// Grow-only device allocation (never reallocated while a stream may use it
+// within one call; growth happens before launches).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(size_t b) {
+        if (b <= bytes) return;
+        release();
+        NLOD_HIP(hipMalloc(&p, b));
+        bytes = b;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+enum Precision : int { PREC_FP32 = 0, PREC_FP16 = 1 };
+enum GradMode : int { GRAD_IGNORE = 0, GRAD_OVERWRITE = 1, GRAD_ACCUMULATE = 2 };
+
+struct Model;
+
+// Saved state of one forward pass (tcnn's ForwardContext).
+struct ForwardCtx {
+    uint32_t N = 0;
+    bool use_inference_params = false;
+    bool prepare_input_gradients = false;
+    bool recompute = false;  // activations were not stored (FP16 path, large batches)
+    DevBuf enc;     // FP32: encoded SoA float [K][N]; FP16: encoded half [N/16 tiles]...
+    DevBuf hidden;  // FP32: SoA float [L][W][N]; FP16: half tiles
+    DevBuf out;     // post-activation outputs float [N][n_output]
+};
+
+struct Model {
+    Layout L;
+    int precision = PREC_FP32;
+    const float* params = nullptr;
+    const float* params_inference = nullptr;
+    // FP16 MFMA weights (network part), packed for the fused kernels, and the
+    // device copy of the encoding description they read
+    DevBuf packed, packed_inference, program;
+    // scratch
+    DevBuf ws_enc, ws_delta, ws_delta_out, ws_denc, ws_wgrad;
+
+    Model(uint32_t n_in, uint32_t n_out, const Json& enc, const Json& net, int prec);
+
+    void set_params(hipStream_t s, const float* p, const float* p_inf);
+    void require_params(bool inference) const;
+    const float* weights(bool inference) const { return inference ? params_inference : params; }
+
+    void encode(hipStream_t s, uint32_t N, const float* in, float* enc_soa, bool inference);
+    void inference(hipStream_t s, uint32_t N, const float* in, float* out);
+    std::unique_ptr<ForwardCtx> forward(hipStream_t s, uint32_t N, const float* in, float* out, bool use_inference,
+                                        bool prepare_input_gradients);
+    void backward(hipStream_t s, ForwardCtx& ctx, const float* in, const float* d_out, float* d_params,
+                  float* d_in, int grad_mode);
+    void init_params_host(uint32_t seed, float* out) const;
+};
+
+// ---- mlp_mfma.hip (FP16 MFMA form) ----------------------------------------------
+bool mfma_supported(const Layout& L);
+size_t mfma_packed_bytes(const NetShape& net);
+size_t mfma_program_bytes();
+size_t mfma_fused_smem_bytes(const NetShape& net);
+// encoding description for the fused kernels, copied to device memory (synchronous)
+void mfma_upload_program(const Layout& L, void* d_program);
+void mfma_pack_weights(hipStream_t stream, const NetShape& net, const float* weights, void* packed);
+// fused encode + MLP inference
+void mfma_fused_inference(hipStream_t stream, const Layout& L, const void* d_program, const float* params,
+                          const void* packed, const float* in, uint32_t N, float* out);
+
+}  // namespace nlod

@@ -1,0 +1,207 @@
+"""tcnn-style operator surface over libnlod (host side of the C ABI).
+
+Mirrors the parts of tiny-cuda-nn the neural-LoD path uses:
+``NetworkWithInputEncoding`` (include/tiny-cuda-nn/network_with_input_encoding.h)
+with inference / forward / backward, and a ``torch.autograd.Function``
+harness in the style of tinycudann's ``_module_function``
+(bindings/torch/tinycudann/modules.py:59-128): gradients of the output are
+scaled by ``loss_scale`` before the native backward and divided afterwards.
+
+Device memory and streams come from torch; every computation runs in
+libnlod's HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import ModelInfo
+
+FP32 = 0
+FP16 = 1
+GRAD_IGNORE, GRAD_OVERWRITE, GRAD_ACCUMULATE = 0, 1, 2
+
+
+def _stream(device=None):
+    return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _ptr(t: torch.Tensor | None):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+def _as_json(cfg) -> bytes:
+    return (cfg if isinstance(cfg, str) else json.dumps(cfg)).encode()
+
+
+def _check_f32(t: torch.Tensor, name: str, shape=None):
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+        raise TypeError(f"{name} must be a contiguous float32 CUDA tensor")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+
+
+class Context:
+    """Saved forward state (owned by libnlod; freed with the object)."""
+
+    def __init__(self, handle: C.c_void_p, model: "NetworkWithInputEncoding", n: int):
+        self._h = handle
+        self.model = model
+        self.n = n
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().nlod_ctx_destroy(self._h)
+            self._h = None
+
+
+class NetworkWithInputEncoding:
+    """Model = encoding (hash grid / SH / identity composites) + fully fused MLP.
+
+    Parameters are one float32 vector (network matrices first, then encoding
+    tables) bound with :meth:`set_params`; the model keeps a reference to the
+    bound tensors."""
+
+    def __init__(self, n_input_dims: int, n_output_dims: int, encoding_config, network_config,
+                 precision: int = FP32):
+        h = C.c_void_p()
+        _lib.call("nlod_model_create", n_input_dims, n_output_dims, _as_json(encoding_config),
+                  _as_json(network_config), precision, C.byref(h))
+        self._h = h
+        info = ModelInfo()
+        _lib.call("nlod_model_get_info", h, C.byref(info))
+        self.n_params = info.n_params
+        self.n_network_params = info.n_network_params
+        self.n_input_dims = info.input_width
+        self.encoded_width = info.encoded_width
+        self.padded_output_width = info.padded_output_width
+        self.n_output_dims = info.output_width
+        self.network_width = info.network_width
+        self.n_hidden_layers = info.n_hidden_layers
+        self.precision = info.precision
+        self._params = None
+        self._inference_params = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().nlod_model_destroy(self._h)
+            self._h = None
+
+    # -- layout / parameters ----------------------------------------------------
+    def describe(self) -> dict:
+        need = C.c_size_t()
+        _lib.call("nlod_model_describe", self._h, None, 0, C.byref(need))
+        buf = C.create_string_buffer(need.value)
+        _lib.call("nlod_model_describe", self._h, buf, need.value, C.byref(need))
+        return json.loads(buf.value.decode())
+
+    def initial_params(self, seed: int = 1337) -> np.ndarray:
+        """tcnn Trainer initialisation (host float32[n_params])."""
+        out = np.zeros(self.n_params, np.float32)
+        _lib.call("nlod_model_init_params", self._h, seed, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def set_params(self, params: torch.Tensor, inference_params: torch.Tensor | None = None) -> None:
+        _check_f32(params, "params", (self.n_params,))
+        if inference_params is not None:
+            _check_f32(inference_params, "inference_params", (self.n_params,))
+        _lib.call("nlod_model_set_params", self._h, _stream(params.device), _ptr(params), _ptr(inference_params))
+        self._params = params
+        self._inference_params = inference_params
+
+    # -- compute ------------------------------------------------------------------
+    def inference(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        _check_f32(x, "input")
+        n = x.shape[0]
+        if x.dim() != 2 or x.shape[1] != self.n_input_dims:
+            raise ValueError(f"input must be [n, {self.n_input_dims}]")
+        if out is None:
+            out = torch.empty((n, self.n_output_dims), dtype=torch.float32, device=x.device)
+        _check_f32(out, "output", (n, self.n_output_dims))
+        _lib.call("nlod_model_inference", self._h, _stream(x.device), n, _ptr(x), _ptr(out))
+        return out
+
+    def encode(self, x: torch.Tensor, use_inference_params: bool = True) -> torch.Tensor:
+        """Encoded features as [encoded_width, n] (SoA)."""
+        _check_f32(x, "input")
+        n = x.shape[0]
+        enc = torch.empty((self.encoded_width, n), dtype=torch.float32, device=x.device)
+        _lib.call("nlod_model_encode", self._h, _stream(x.device), n, _ptr(x), _ptr(enc), int(use_inference_params))
+        return enc
+
+    def encode_backward(self, x: torch.Tensor, d_encoded: torch.Tensor, d_params: torch.Tensor | None = None,
+                        mode: int = GRAD_OVERWRITE) -> torch.Tensor:
+        _check_f32(x, "input")
+        _check_f32(d_encoded, "d_encoded", (self.encoded_width, x.shape[0]))
+        if d_params is None:
+            d_params = torch.empty(self.n_params, dtype=torch.float32, device=x.device)
+        _lib.call("nlod_model_encode_backward", self._h, _stream(x.device), x.shape[0], _ptr(x), _ptr(d_encoded),
+                  _ptr(d_params), mode)
+        return d_params
+
+    def forward(self, x: torch.Tensor, use_inference_params: bool = False,
+                prepare_input_gradients: bool = False) -> tuple[torch.Tensor, Context]:
+        _check_f32(x, "input")
+        n = x.shape[0]
+        out = torch.empty((n, self.n_output_dims), dtype=torch.float32, device=x.device)
+        h = C.c_void_p()
+        _lib.call("nlod_model_forward", self._h, _stream(x.device), n, _ptr(x), _ptr(out), int(use_inference_params),
+                  int(prepare_input_gradients), C.byref(h))
+        return out, Context(h, self, n)
+
+    def backward(self, ctx: Context, x: torch.Tensor, d_out: torch.Tensor, d_params: torch.Tensor | None = None,
+                 input_gradients: bool = False, mode: int = GRAD_OVERWRITE):
+        _check_f32(d_out, "d_out", (ctx.n, self.n_output_dims))
+        if d_params is None and mode != GRAD_IGNORE:
+            d_params = torch.empty(self.n_params, dtype=torch.float32, device=x.device)
+        d_in = torch.empty_like(x) if input_gradients else None
+        _lib.call("nlod_model_backward", self._h, _stream(x.device), ctx._h, _ptr(x), _ptr(d_out), _ptr(d_params),
+                  _ptr(d_in), mode)
+        return d_params, d_in
+
+
+class _ModuleFunction(torch.autograd.Function):
+    """Autograd bridge (tinycudann modules.py:59-128 pattern)."""
+
+    @staticmethod
+    def forward(ctx, model: NetworkWithInputEncoding, x, params, loss_scale):
+        ctx.set_materialize_grads(False)
+        model.set_params(params)
+        out, native_ctx = model.forward(x)
+        ctx.model = model
+        ctx.native_ctx = native_ctx
+        ctx.loss_scale = loss_scale
+        ctx.save_for_backward(x, params)
+        return out
+
+    @staticmethod
+    def backward(ctx, d_out):
+        if d_out is None:
+            return None, None, None, None
+        x, params = ctx.saved_tensors
+        model = ctx.model
+        want_in = ctx.needs_input_grad[1]
+        scaled = (d_out * ctx.loss_scale).contiguous()
+        d_params, d_in = model.backward(ctx.native_ctx, x, scaled, input_gradients=want_in)
+        d_params = d_params / ctx.loss_scale
+        if d_in is not None:
+            d_in = d_in / ctx.loss_scale
+        return None, d_in, d_params, None
+
+
+class Module(torch.nn.Module):
+    """torch.nn.Module wrapper: parameters are one float32 vector."""
+
+    def __init__(self, n_input_dims, n_output_dims, encoding_config, network_config, seed=1337,
+                 precision=FP32, loss_scale=1.0):
+        super().__init__()
+        self.native = NetworkWithInputEncoding(n_input_dims, n_output_dims, encoding_config, network_config, precision)
+        self.params = torch.nn.Parameter(torch.from_numpy(self.native.initial_params(seed)))
+        self.loss_scale = loss_scale
+
+    def forward(self, x):
+        return _ModuleFunction.apply(self.native, x.contiguous(), self.params, self.loss_scale)

@@ -1,0 +1,272 @@
+"""Pins the encoding/MLP oracle (oracle/model_oracle.c) against independent
+mathematics and library routines:
+
+* spherical harmonics == scipy.special's complex Y_l^m turned real, and the
+  basis is orthonormal under Lebedev-free Monte-Carlo quadrature;
+* a dense grid level == textbook trilinear interpolation
+  (scipy.interpolate.RegularGridInterpolator) at x*scale + 0.5;
+  weights sum to one (constant table -> constant); vertex queries return the
+  vertex value; coherent-prime hash known answers;
+* MLP forward == a numpy matmul chain; backward == central finite
+  differences of the FP64 forward (parameters of every matrix and grid level);
+* BCE / relative-L2 closed forms; Adam == torch.optim.Adam (tiny epsilon) and
+  the zero-gradient skip rule for non-matrix parameters.
+"""
+import numpy as np
+import pytest
+import scipy.special
+from scipy.interpolate import RegularGridInterpolator
+
+from neural_lod_amd import configs
+from oracle import model as M
+
+NET = {"otype": "FullyFusedMLP", "n_neurons": 16, "n_hidden_layers": 2, "activation": "ReLU",
+       "output_activation": "None"}
+
+
+def _sh_layout(deg):
+    return M.Layout(3, 1, {"otype": "SphericalHarmonics", "degree": deg}, NET)
+
+
+def _real_sh_scipy(l, m, d):
+    theta = np.arccos(np.clip(d[:, 2], -1, 1))
+    phi = np.arctan2(d[:, 1], d[:, 0])
+    try:
+        Y = scipy.special.sph_harm_y(l, abs(m), theta, phi)
+    except AttributeError:  # older scipy
+        Y = scipy.special.sph_harm(abs(m), l, phi, theta)
+    # scipy's Y_l^m carries the Condon-Shortley phase, which the real basis keeps
+    if m == 0:
+        return Y.real
+    if m > 0:
+        return np.sqrt(2) * Y.real
+    return np.sqrt(2) * Y.imag
+
+
+@pytest.mark.parametrize("deg", [1, 4, 8])
+def test_sh_matches_scipy(deg):
+    rng = np.random.default_rng(0)
+    d = rng.normal(size=(500, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    x = ((d + 1) / 2).astype(np.float32)
+    dd = x.astype(np.float64) * 2 - 1  # what the encoding sees (float32 rounding of the input)
+    L = _sh_layout(deg)
+    enc = M.encode(L, np.zeros(L.n_params, np.float32), x)
+    pad = L.segs[0].n_pad  # SH padding rows (1.0) come first
+    assert (enc[:, :pad] == 1.0).all()
+    enc = enc[:, pad:]
+    for l in range(deg):
+        for m in range(-l, l + 1):
+            ref = _real_sh_scipy(l, m, dd / np.linalg.norm(dd, axis=1, keepdims=True))
+            # the polynomial form is evaluated at the (nearly) unit vector dd itself
+            got = enc[:, l * l + l + m]
+            assert np.allclose(got, ref, atol=2e-6 * (l + 1) ** 2), (l, m)
+
+
+def test_sh_low_order_closed_forms():
+    """Sign/order convention of tiny-cuda-nn (spherical_harmonics.h:72-84 comments)."""
+    rng = np.random.default_rng(7)
+    d = rng.normal(size=(100, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    x = ((d + 1) / 2).astype(np.float32)
+    v = x.astype(np.float64) * 2 - 1
+    X, Y, Z = v[:, 0], v[:, 1], v[:, 2]
+    sp = np.sqrt(np.pi)
+    ref = [np.full_like(X, 1 / (2 * sp)), -np.sqrt(3) * Y / (2 * sp), np.sqrt(3) * Z / (2 * sp),
+           -np.sqrt(3) * X / (2 * sp), np.sqrt(15) * X * Y / (2 * sp), -np.sqrt(15) * Y * Z / (2 * sp),
+           np.sqrt(5) * (3 * Z * Z - 1) / (4 * sp), -np.sqrt(15) * X * Z / (2 * sp),
+           np.sqrt(15) * (X * X - Y * Y) / (4 * sp)]
+    L = _sh_layout(4)
+    enc = M.encode(L, np.zeros(L.n_params, np.float32), x)
+    for k, r in enumerate(ref):
+        assert np.allclose(enc[:, k], r, atol=1e-12), k
+
+
+def test_sh_orthonormal():
+    rng = np.random.default_rng(1)
+    d = rng.normal(size=(400000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    L = _sh_layout(4)
+    enc = M.encode(L, np.zeros(L.n_params, np.float32), ((d + 1) / 2).astype(np.float32))
+    enc = enc[:, L.segs[0].n_pad:L.segs[0].n_pad + 16]
+    gram = enc.T @ enc * (4 * np.pi / d.shape[0])
+    assert np.allclose(gram, np.eye(16), atol=2e-2)
+
+
+def _grid_layout(levels=4, base=4, scale=1.5, log2T=19, F=2):
+    enc = {"otype": "HashGrid", "n_levels": levels, "n_features_per_level": F, "log2_hashmap_size": log2T,
+           "base_resolution": base, "per_level_scale": scale}
+    return M.Layout(3, 1, enc, NET)
+
+
+def test_dense_level_is_trilinear():
+    L = _grid_layout(levels=3, base=6, scale=1.5)
+    offs, scales, res = L.grid_levels(0)
+    rng = np.random.default_rng(2)
+    params = rng.uniform(-1, 1, L.n_params).astype(np.float32)
+    x = rng.uniform(0, 1, (2000, 3)).astype(np.float32)
+    enc = M.encode(L, params, x)
+    for lvl in range(3):
+        r = int(res[lvl])
+        assert r ** 3 <= 2 ** 19  # dense level
+        tab = params[L.n_net_params + offs[lvl] * 2: L.n_net_params + offs[lvl] * 2 + r ** 3 * 2].reshape(r, r, r, 2)
+        # index x + y*r + z*r^2  ->  tab[z][y][x]
+        pos = x.astype(np.float64) * np.float64(scales[lvl]) + 0.5
+        inside = np.all(pos < r - 1, axis=1)  # corners inside the table (no wrap-around)
+        grid = np.arange(r, dtype=np.float64)
+        for f in range(2):
+            interp = RegularGridInterpolator((grid, grid, grid), tab[..., f].astype(np.float64))
+            ref = interp(pos[inside][:, ::-1])  # (z, y, x) order
+            assert np.allclose(enc[inside, lvl * 2 + f], ref, atol=1e-6), lvl
+
+
+def test_grid_partition_of_unity_and_vertices():
+    L = _grid_layout(levels=6, base=8, scale=2.0, log2T=12)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0, 1, (3000, 3)).astype(np.float32)
+    params = np.zeros(L.n_params, np.float32)
+    params[L.n_net_params:] = 0.37
+    enc = M.encode(L, params, x)
+    assert np.allclose(enc[:, :12], 0.37, atol=1e-12)
+    # vertex query on level 0 (scale 7): x*7 + 0.5 = 3  ->  x = 2.5/7 (float32)
+    offs, scales, res = L.grid_levels(0)
+    p2 = rng.uniform(-1, 1, L.n_params).astype(np.float32)
+    xv = np.float32(2.5) / np.float32(7.0)
+    q = np.array([[xv, xv, xv]], np.float32)
+    e = M.encode(L, p2, q)
+    idx = 3 + 3 * 8 + 3 * 64
+    ref = p2[L.n_net_params + idx * 2: L.n_net_params + idx * 2 + 2]
+    assert np.allclose(e[0, :2], ref, atol=1e-5)
+
+
+def test_hash_known_answers():
+    L = _grid_layout(levels=8, base=16, scale=2.0, log2T=10)
+    seg = L.segs[0]
+    offs, scales, res = L.grid_levels(0)
+    lvl = 4  # res 256: hashed into 2^10 entries
+    assert res[lvl] ** 3 > 2 ** 10
+    # choose x so that cell = (1, 2, 3) at this level: x*scale + 0.5 in [c, c+1)
+    x = np.array([(c + 0.25) / scales[lvl] for c in (1, 2, 3)], np.float32)
+    idx = M.grid_corners(seg, x)[lvl]
+    # corner 0 = cell (1,2,3): 1*1 xor 2*2654435761 xor 3*805459861 (mod 2^32) mod 2^10
+    expect0 = ((1 * 1) ^ ((2 * 2654435761) % 2 ** 32) ^ ((3 * 805459861) % 2 ** 32)) % 1024
+    expect7 = ((2 * 1) ^ ((3 * 2654435761) % 2 ** 32) ^ ((4 * 805459861) % 2 ** 32)) % 1024
+    assert idx[0] == expect0 == 476
+    assert idx[7] == expect7
+
+
+def test_mlp_forward_matches_numpy():
+    cfg = configs.VISIBILITY
+    L = M.Layout(6, 1, cfg["encoding"], cfg["network"])
+    rng = np.random.default_rng(4)
+    p = M.xavier_uniform(L, 4, grid_range=1.0)
+    x = rng.uniform(0, 1, (300, 6)).astype(np.float32)
+    out, enc, hidden = M.forward(L, p, x, return_hidden=True)
+    n = L.net
+    W, K = n.width, n.n_input
+    w = p.astype(np.float64)
+    mats = [w[:W * K].reshape(W, K)]
+    off = W * K
+    for _ in range(n.n_hidden_layers - 1):
+        mats.append(w[off: off + W * W].reshape(W, W))
+        off += W * W
+    mats.append(w[off: off + 16 * W].reshape(16, W))
+    h = enc.T
+    for k, Wm in enumerate(mats[:-1]):
+        h = np.maximum(Wm @ h, 0)
+        assert np.allclose(hidden[:, k, :], h.T, rtol=1e-12, atol=1e-12)
+    y = 1 / (1 + np.exp(-(mats[-1] @ h)))
+    assert np.allclose(out, y.T, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["visibility", "throughput"])
+def test_backward_matches_finite_differences(name):
+    cfg = configs.ALL[name]
+    L = M.Layout(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"])
+    rng = np.random.default_rng(5)
+    p = M.xavier_uniform(L, 5, grid_range=0.5)
+    x = rng.uniform(0.02, 0.98, (24, cfg["n_input_dims"])).astype(np.float32)
+    if name == "throughput":
+        x[:, 9] = rng.integers(0, 5, 24)
+    d_out = np.zeros((24, L.net.n_output_padded))
+    d_out[:, :L.net.n_output] = rng.normal(size=(24, L.net.n_output))
+    g, _ = M.backward(L, p, x, d_out)
+
+    def loss(pp):
+        o = M.forward(L, pp, x)
+        return float((o * d_out).sum())
+
+    # central differences of the FP64 forward; parameters are float32, so the
+    # actual step is taken from the rounded perturbed values
+    idx_net = rng.choice(L.n_net_params, 40, replace=False)
+    # grid entries actually touched by the queries: pick from those with large gradient
+    grid_idx = np.nonzero(np.abs(g[L.n_net_params:]) > 1e-3 * np.abs(g[L.n_net_params:]).max())[0]
+    idx_grid = L.n_net_params + rng.choice(grid_idx, 40, replace=False)
+    for i in np.concatenate([idx_net, idx_grid]):
+        h = 1e-4
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (loss(pp) - loss(pm)) / (float(pp[i]) - float(pm[i]))
+        assert abs(fd - g[i]) <= 5e-4 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_losses_closed_form():
+    pred = np.array([[0.7, 0.0], [0.2, 0.0], [1.0, 0.0]])
+    target = np.array([[1.0], [0.0], [1.0]], np.float32)
+    v, g = M.loss("BinaryCrossEntropy", pred, target, 1, loss_scale=128.0)
+    n = 3.0
+    assert np.isclose(v[0, 0], -np.log(0.7) / n)
+    assert np.isclose(v[1, 0], -np.log(0.8) / n)
+    assert np.isclose(v[2, 0], -np.log(1 - 1e-5) / n)  # prediction clamped to 1-1e-5
+    assert np.isclose(g[0, 0], 128.0 * (0.7 - 1.0) / (0.7 * 0.3) / n)
+    assert v[0, 1] == 0 and g[0, 1] == 0  # padded output column
+    # relative L2: (p-t)^2 / (p^2 + 0.01), gradient treats the denominator as constant
+    pred2 = np.array([[2.0, 0.5, 0.0]])
+    t2 = np.array([[1.0, 1.5]], np.float32)
+    v2, g2 = M.loss("RelativeL2", pred2, t2, 2, loss_scale=1.0)
+    assert np.isclose(v2[0, 0], 1.0 / 4.01 / 2)
+    assert np.isclose(v2[0, 1], 1.0 / 0.26 / 2)
+    assert np.isclose(g2[0, 0], 2 * 1.0 / 4.01 / 2)
+    assert np.isclose(g2[0, 1], 2 * -1.0 / 0.26 / 2)
+
+
+def test_adam_matches_torch():
+    import torch
+    rng = np.random.default_rng(6)
+    n, n_matrix = 64, 40
+    w0 = rng.normal(size=n).astype(np.float32)
+    grads = [rng.normal(size=n).astype(np.float32) for _ in range(3)]
+    grads[1][50:60] = 0.0  # non-matrix params with zero gradient are skipped
+    lr, b1, b2, eps, l2 = 1e-2, 0.9, 0.99, 1e-15, 1e-3
+    w = w0.copy()
+    m = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    steps = np.zeros(n, np.uint32)
+    for gr in grads:
+        M.adam_step(w, (gr * 128.0).astype(np.float32), m, v, steps, n_matrix, 128.0, lr, b1, b2, eps, l2)
+    assert steps[:50].tolist() == [3] * 50 and steps[50:60].tolist() == [2] * 10
+    # torch: matrix params with weight decay (= tcnn l2_reg), non-matrix without
+    tm = torch.tensor(w0[:n_matrix].astype(np.float64), requires_grad=True)
+    opt = torch.optim.Adam([tm], lr=lr, betas=(b1, b2), eps=eps, weight_decay=l2)
+    for gr in grads:
+        tm.grad = torch.tensor(gr[:n_matrix].astype(np.float64))
+        opt.step()
+    assert np.allclose(w[:n_matrix], tm.detach().numpy(), rtol=1e-5, atol=1e-6)
+    # params 60.. saw all three gradients, no weight decay
+    tn = torch.tensor(w0[60:].astype(np.float64), requires_grad=True)
+    opt2 = torch.optim.Adam([tn], lr=lr, betas=(b1, b2), eps=eps)
+    for gr in grads:
+        tn.grad = torch.tensor(gr[60:].astype(np.float64))
+        opt2.step()
+    assert np.allclose(w[60:], tn.detach().numpy(), rtol=1e-5, atol=1e-6)
+
+
+def test_f32_loop_close_to_fp64():
+    cfg = configs.VISIBILITY
+    L = M.Layout(6, 1, cfg["encoding"], cfg["network"])
+    p = M.xavier_uniform(L, 8, grid_range=1.0)
+    x = np.random.default_rng(8).uniform(0, 1, (2000, 6)).astype(np.float32)
+    ref = M.forward(L, p, x)[:, :1]
+    got = M.infer_f32(L, p, x)
+    assert np.abs(got - ref).max() < 1e-5
