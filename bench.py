@@ -1,0 +1,152 @@
+"""Benchmark of the neural-LoD hot path on one MI355X.
+
+Headline (the JSON line's `value`): queries/s of the FP16-MFMA fused
+encode + MLP inference of the visibility network (16-level hash grid, 2 FP32
+features/level, 2^19 entries/level, base 16, growth 1.5, + degree-4 SH of the
+direction; MLP 64 wide, 4 hidden ReLU layers, sigmoid output) on a batch of
+2^20 synthetic queries resident in HBM.  One step = one inference of the
+whole batch.  The same run also measures the plain FP32 GPU form, the error
+of both against the FP64 CPU oracle, and a CPU baseline (the oracle's
+straightforward FP32 C loop, OpenMP over queries) on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+N_QUERIES = 1 << 20
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# algorithmic bytes per query of the fused inference: 16 levels x 8 corners x
+# 2 features x 4 B gathered + 6 x 4 B input + 1 x 4 B output
+BYTES_PER_QUERY = 16 * 8 * 2 * 4 + 6 * 4 + 4
+FLOP_PER_QUERY = 2 * (48 * 64 + 3 * 64 * 64 + 64 * 1)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from neural_lod_amd import configs, synthetic
+    from neural_lod_amd.tcnn import FP16, FP32, NetworkWithInputEncoding
+    from oracle import model as M
+
+    dev = torch.device("cuda:0")
+    cfg = configs.VISIBILITY
+    L = M.Layout(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"])
+    # Xavier MLP; grid table drawn U[-1, 1] so the FP16 error check is not vacuous
+    params_np = M.xavier_uniform(L, seed=0, grid_range=1.0)
+    x_np = synthetic.visibility_queries(N_QUERIES, seed=1)
+    params = torch.from_numpy(params_np).to(dev)
+    x = torch.from_numpy(x_np).to(dev)
+    log(f"workload: {N_QUERIES} queries, {L.n_params} params ({L.n_params * 4 / 2**20:.1f} MiB FP32)")
+
+    models = {}
+    for name, prec in (("fp16", FP16), ("fp32", FP32)):
+        m = NetworkWithInputEncoding(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"], prec)
+        m.set_params(params)
+        models[name] = m
+    outs = {k: torch.empty((N_QUERIES, 1), device=dev) for k in models}
+
+    def run(name, k):
+        m, o = models[name], outs[name]
+        for _ in range(k):
+            m.inference(x, o)
+
+    results = {}
+    for name in ("fp32", "fp16"):
+        run(name, args.warmup)
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        run(name, args.steps)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = e0.elapsed_time(e1) / args.steps
+        results[name] = {"ms": ms, "qps": N_QUERIES / (ms * 1e-3), "wall_s": wall}
+        log(f"{name}: {ms:.4f} ms/step  {N_QUERIES / ms / 1e6:.2f} Mq/ms...")
+
+    # accuracy vs the FP64 oracle on a sample of the same batch
+    n_chk = 1 << 14
+    ref = M.forward(L, params_np, x_np[:n_chk])[:, :1]
+    err = {k: float(np.abs(outs[k][:n_chk].cpu().numpy() - ref).max()) for k in outs}
+    log(f"max abs error vs FP64 oracle ({n_chk} queries): {err}")
+
+    # CPU baseline: the oracle's FP32 C loop on a bounded sample
+    cpu = None
+    if not args.no_cpu:
+        threads = M._lib().nlo_omp_threads()
+        n_cpu = 1 << 17
+        xs = x_np[:n_cpu]
+        M.infer_f32(L, params_np, xs[:1024])  # warm caches / threads
+        t0 = time.perf_counter()
+        reps = 0
+        while True:
+            M.infer_f32(L, params_np, xs)
+            reps += 1
+            if time.perf_counter() - t0 > 10.0:
+                break
+        dt = time.perf_counter() - t0
+        cpu = {"value": n_cpu * reps / dt, "unit": "queries/s", "cores": int(threads), "kind": "port",
+               "sample": f"{reps} x {n_cpu} queries of the same workload (FP32 C loop, OpenMP over queries)"}
+        log(f"cpu baseline: {cpu['value']:.3e} queries/s on {threads} threads")
+
+    fp16 = results["fp16"]
+    achieved = BYTES_PER_QUERY * N_QUERIES / (fp16["ms"] * 1e-3) / 1e9
+    line = {
+        "metric": "encode+MLP queries/s & train steps/s, FP16-MFMA vs FP32 plain, <=2e-3 err",
+        "value": fp16["qps"],
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fp16["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16-mfma/fp32-acc",
+        "data": "synthetic",
+        "config": {"workload": "visibility network inference: hash grid 16x2 (2^19/level, base 16, x1.5) + SH4, "
+                               "MLP 64x4 ReLU, sigmoid; batch 2^20 queries",
+                   "global_batch": N_QUERIES, "parallelism": "single GPU"},
+        "plain_fp32": {"value": results["fp32"]["qps"], "ms_per_step": results["fp32"]["ms"],
+                       "max_abs_err_vs_fp64": err["fp32"]},
+        "fp16_max_abs_err_vs_fp64": err["fp16"],
+        "speedup_vs_plain_fp32": results["fp32"]["ms"] / fp16["ms"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fused_inference_kernel<64,2,4>",
+                     "bytes_per_query": BYTES_PER_QUERY, "flop_per_query": FLOP_PER_QUERY},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

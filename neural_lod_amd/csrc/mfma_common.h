@@ -1,0 +1,263 @@
+// Shared pieces of the FP16-MFMA kernels: the device-side encoding program,
+// the per-lane feature-window encoder, the packed weight-fragment layout and
+// the register-resident MLP tail.  See mlp_mfma.hip for the operand layout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "device_common.h"
+#include "layout.h"
+#include "nlod_common.h"
+#include "sh_basis.h"
+
+namespace nlod {
+
+constexpr int kMaxSegs = 8;
+
+// Encoding description read by the fused kernels (uniform, from device memory).
+struct DevSegment {
+    int32_t type;
+    uint32_t in_offset, out_offset, n_out, n_pad, degree;
+    float id_scale, id_offset;
+    uint64_t param_offset;
+    GridLevels lv;
+};
+struct EncProgram {
+    uint32_t n_segs, n_input_dims, enc_width, k_padded;
+    DevSegment segs[kMaxSegs];
+};
+
+inline uint32_t k_padded_of(const NetShape& n) { return (n.n_input + 31) / 32 * 32; }
+
+// Packed FP16 weight fragments: per matrix, per 16-row tile, per 32-column
+// chunk, 64 lanes x 8 halves (layer 0 in natural column order, later layers
+// with columns permuted to the accumulator layout).
+inline size_t frag_count(uint32_t rows, uint32_t cols_padded) { return (size_t)(rows / 16) * (cols_padded / 32); }
+
+inline size_t packed_halves(const NetShape& n) {
+    const uint32_t W = n.width;
+    const size_t frags = frag_count(W, k_padded_of(n)) + (size_t)(n.n_hidden_layers - 1) * frag_count(W, W) +
+                         frag_count(16, W);
+    return frags * 64 * 8;
+}
+
+// k-slot (g, j) of chunk c of a layer fed by the previous layer's accumulators
+// -> hidden unit: slots 0-3 come from accumulator tile 2c, 4-7 from tile 2c+1.
+__host__ __device__ inline uint32_t permuted_unit(uint32_t c, uint32_t g, uint32_t j) {
+    return 32u * c + (j < 4 ? 4u * g + j : 16u + 4u * g + (j - 4u));
+}
+
+// One level of a grid for one query: F features interpolated in FP32.
+template <int F>
+__device__ __forceinline__ void grid_level(const GridLevels& lv, uint32_t level, float x0, float x1, float x2,
+                                           const float* __restrict__ table, float* v) {
+    const float scale = lv.scale[level];
+    const uint32_t res = lv.res[level];
+    const uint32_t off = lv.offset[level];
+    const uint32_t hs = lv.offset[level + 1] - off;
+    const float* tab = table + (size_t)off * F;
+    float fr[3];
+    uint32_t cell[3];
+    grid_pos(x0, scale, lv.interp, fr[0], cell[0]);
+    grid_pos(x1, scale, lv.interp, fr[1], cell[1]);
+    grid_pos(x2, scale, lv.interp, fr[2], cell[2]);
+#pragma unroll
+    for (int f = 0; f < F; ++f) v[f] = 0.0f;
+    if (lv.interp == INTERP_NEAREST) {
+        const uint32_t idx = grid_index(lv.grid_type, hs, res, cell[0], cell[1], cell[2]);
+#pragma unroll
+        for (int f = 0; f < F; ++f) v[f] = tab[(size_t)idx * F + f];
+        return;
+    }
+    float vals[8][F];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const uint32_t idx = grid_index(lv.grid_type, hs, res, cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
+                                        cell[2] + ((c >> 2) & 1));
+        if constexpr (F == 2) {
+            const float2 t2 = *reinterpret_cast<const float2*>(tab + (size_t)idx * 2);
+            vals[c][0] = t2.x;
+            vals[c][1] = t2.y;
+        } else if constexpr (F >= 4) {
+#pragma unroll
+            for (int f = 0; f < F; f += 4) {
+                const float4 t4 = *reinterpret_cast<const float4*>(tab + (size_t)idx * F + f);
+                vals[c][f] = t4.x;
+                vals[c][f + 1] = t4.y;
+                vals[c][f + 2] = t4.z;
+                vals[c][f + 3] = t4.w;
+            }
+        } else {
+            vals[c][0] = tab[idx];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        float w = 1.0f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) w *= ((c >> d) & 1) ? fr[d] : 1.0f - fr[d];
+#pragma unroll
+        for (int f = 0; f < F; ++f) v[f] += w * vals[c][f];
+    }
+}
+
+// win[j] = v for a runtime window slot j, via selects (no dynamic register indexing).
+__device__ __forceinline__ void place(float* win, int j, float v) {
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) win[jj] = (jj == j) ? v : win[jj];
+}
+
+// Encoded features [w0, w0 + 8) of one query in FP32 (0 beyond the encoded width).
+template <int F, int MAXD>
+__device__ __forceinline__ void encode_window_f32(const EncProgram& P, const float* __restrict__ params,
+                                                  const float* __restrict__ q, uint32_t w0, float* win) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) win[j] = 0.0f;
+    const uint32_t w1 = w0 + 8;
+    for (uint32_t si = 0; si < P.n_segs; ++si) {
+        const DevSegment& s = P.segs[si];
+        const uint32_t r0 = s.out_offset, r1 = s.out_offset + s.n_out + s.n_pad;
+        if (r1 <= w0 || r0 >= w1) continue;
+        const float* x = q + s.in_offset;
+        if (s.type == SEG_GRID) {
+            // a grid segment starts at a multiple of F, so levels tile the window
+            const int l_begin = w0 > r0 ? (int)((w0 - r0) / F) : 0;
+            const int n_levels = (int)s.lv.n_levels;
+            const float x0 = x[0], x1 = x[1], x2 = x[2];
+            const float* table = params + s.param_offset;
+#pragma unroll
+            for (int t = 0; t < 8 / F; ++t) {
+                const int l = l_begin + t;
+                const int row = (int)r0 + l * F - (int)w0;
+                // gather unconditionally (clamped level: indices stay inside it), place if valid
+                float v[F];
+                grid_level<F>(s.lv, (uint32_t)min(l, n_levels - 1), x0, x1, x2, table, v);
+                const bool ok = l < n_levels && row >= 0 && row < 8;
+#pragma unroll
+                for (int f = 0; f < F; ++f) place(win, ok ? row + f : -1, v[f]);
+            }
+        } else if (s.type == SEG_SH) {
+            const uint32_t sh0 = r0 + s.n_pad;  // SH pad rows (1.0) come first
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t row = w0 + j;
+                if (row >= r0 && row < sh0) win[j] = 1.0f;
+            }
+            if (sh0 + s.degree * s.degree > w0 && sh0 < w1) {
+                const float vx = x[0] * 2.f - 1.f, vy = x[1] * 2.f - 1.f, vz = x[2] * 2.f - 1.f;
+                float X[MAXD], Y[MAXD], Z[MAXD];
+                X[0] = Y[0] = Z[0] = 1.0f;
+#pragma unroll
+                for (int k = 1; k < MAXD; ++k) {
+                    X[k] = X[k - 1] * vx;
+                    Y[k] = Y[k - 1] * vy;
+                    Z[k] = Z[k - 1] * vz;
+                }
+                const int base = (int)sh0 - (int)w0;
+                sh_basis<MAXD>(s.degree, X, Y, Z, [&](int k, float v) { place(win, base + k, v); });
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint32_t row = w0 + j;
+                if (row >= r0 && row < r0 + s.n_out) win[j] = x[row - r0] * s.id_scale + s.id_offset;
+                else if (row >= r0 + s.n_out && row < r1) win[j] = 1.0f;
+            }
+        }
+    }
+}
+
+template <int F, int MAXD>
+__device__ __forceinline__ half8 encode_window(const EncProgram& P, const float* __restrict__ params,
+                                               const float* __restrict__ q, uint32_t w0) {
+    float win[8];
+    encode_window_f32<F, MAXD>(P, params, q, w0, win);
+    half8 h;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) h[j] = (half_t)win[j];
+    return h;
+}
+
+__device__ __forceinline__ floatx4 mfma16(const half8& a, const half8& b, const floatx4& c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// Activated accumulator tiles 2c and 2c+1 as the B fragment of chunk c of the next layer.
+__device__ __forceinline__ half8 pack_act(const floatx4& lo, const floatx4& hi, int act) {
+    half8 r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        r[i] = (half_t)act_fwd(act, lo[i]);
+        r[i + 4] = (half_t)act_fwd(act, hi[i]);
+    }
+    return r;
+}
+
+// Hidden layers + output layer of one 16-query tile, from the first layer's
+// accumulators.  Returns the output accumulator (outputs 4g..4g+3 of query lane%16).
+template <int W>
+__device__ __forceinline__ floatx4 mlp_tail16(const half_t* __restrict__ frag, floatx4* acc, uint32_t L, int act) {
+    constexpr int MT = W / 16, KC = W / 32;
+    const int lane = threadIdx.x & 63;
+    half8 b[KC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) b[c] = pack_act(acc[2 * c], acc[2 * c + 1], act);
+    for (uint32_t layer = 1; layer < L; ++layer) {
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            floatx4 s = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+                const half8 a = *reinterpret_cast<const half8*>(frag + ((size_t)(mt * KC + c) * 64 + lane) * 8);
+                s = mfma16(a, b[c], s);
+            }
+            acc[mt] = s;
+        }
+        frag += (size_t)MT * KC * 512;
+#pragma unroll
+        for (int c = 0; c < KC; ++c) b[c] = pack_act(acc[2 * c], acc[2 * c + 1], act);
+    }
+    floatx4 o = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+        const half8 a = *reinterpret_cast<const half8*>(frag + ((size_t)c * 64 + lane) * 8);
+        o = mfma16(a, b[c], o);
+    }
+    return o;
+}
+
+int cu_count();
+
+// Calls f(W, F, MAXD) with integral constants matching the layout.
+template <typename Fn>
+void dispatch_fused(const Layout& L, Fn&& f) {
+    uint32_t F = 2, maxdeg = 0;
+    for (const Segment& s : L.segs) {
+        if (s.type == SEG_GRID) F = s.F;
+        if (s.type == SEG_SH && s.degree > maxdeg) maxdeg = s.degree;
+    }
+    auto with_d = [&](auto Wc, auto Fc) {
+        if (maxdeg <= 4) f(Wc, Fc, std::integral_constant<int, 4>{});
+        else f(Wc, Fc, std::integral_constant<int, 8>{});
+    };
+    auto with_f = [&](auto Wc) {
+        switch (F) {
+            case 1: with_d(Wc, std::integral_constant<int, 1>{}); break;
+            case 2: with_d(Wc, std::integral_constant<int, 2>{}); break;
+            case 4: with_d(Wc, std::integral_constant<int, 4>{}); break;
+            case 8: with_d(Wc, std::integral_constant<int, 8>{}); break;
+            default: fail(NLOD_ERR_UNSUPPORTED, "unsupported n_features_per_level");
+        }
+    };
+    switch (L.net.width) {
+        case 32: with_f(std::integral_constant<int, 32>{}); break;
+        case 64: with_f(std::integral_constant<int, 64>{}); break;
+        case 128: with_f(std::integral_constant<int, 128>{}); break;
+        default: fail(NLOD_ERR_UNSUPPORTED, "FP16 MFMA kernels support widths 32, 64 and 128");
+    }
+}
+
+}  // namespace nlod

@@ -1,0 +1,88 @@
+// FP16 weight packing and the device copy of the encoding program for the
+// MFMA kernels (layouts documented in mfma_common.h).
+#include <hip/hip_runtime.h>
+
+#include "mfma_common.h"
+#include "model.h"
+
+namespace nlod {
+namespace {
+
+__global__ void pack_kernel(uint32_t W, uint32_t K, uint32_t Kp, uint32_t L, const float* __restrict__ w,
+                            half_t* __restrict__ out, uint32_t total_halves) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total_halves) return;
+    const uint32_t j = e & 7u, lane = (e >> 3) & 63u, frag = e >> 9;
+    const uint32_t g = lane >> 4, r16 = lane & 15u;
+    const uint32_t f0 = (W / 16) * (Kp / 32);
+    const uint32_t fh = (W / 16) * (W / 32);
+    float v = 0.0f;
+    if (frag < f0) {  // input layer: natural column order, zero beyond K
+        const uint32_t mt = frag / (Kp / 32), c = frag % (Kp / 32);
+        const uint32_t row = 16 * mt + r16, col = 32 * c + 8 * g + j;
+        v = col < K ? w[(size_t)row * K + col] : 0.0f;
+    } else if (frag < f0 + (L - 1) * fh) {  // hidden matrices: permuted columns
+        const uint32_t f = frag - f0, layer = 1 + f / fh, ff = f % fh;
+        const uint32_t mt = ff / (W / 32), c = ff % (W / 32);
+        const uint32_t row = 16 * mt + r16, col = permuted_unit(c, g, j);
+        v = w[(size_t)W * K + (size_t)(layer - 1) * W * W + (size_t)row * W + col];
+    } else {  // output matrix (16 padded rows): permuted columns
+        const uint32_t c = frag - f0 - (L - 1) * fh;
+        const uint32_t row = r16, col = permuted_unit(c, g, j);
+        v = w[(size_t)W * K + (size_t)(L - 1) * W * W + (size_t)row * W + col];
+    }
+    out[e] = (half_t)v;
+}
+
+EncProgram build_program(const Layout& L) {
+    EncProgram P{};
+    P.n_segs = (uint32_t)L.segs.size();
+    P.n_input_dims = L.n_input_dims;
+    P.enc_width = L.enc_width;
+    P.k_padded = k_padded_of(L.net);
+    for (size_t i = 0; i < L.segs.size(); ++i) {
+        const Segment& s = L.segs[i];
+        DevSegment& d = P.segs[i];
+        d.type = s.type;
+        d.in_offset = s.in_offset;
+        d.out_offset = s.out_offset;
+        d.n_out = s.n_out;
+        d.n_pad = s.n_pad;
+        d.degree = s.degree;
+        d.id_scale = s.id_scale;
+        d.id_offset = s.id_offset;
+        d.param_offset = s.param_offset;
+        d.lv = s.levels;
+    }
+    return P;
+}
+
+}  // namespace
+
+int cu_count() {
+    static int n = [] {
+        int dev = 0, c = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 256;
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) return 256;
+        return c;
+    }();
+    return n;
+}
+
+size_t mfma_packed_bytes(const NetShape& net) { return packed_halves(net) * sizeof(half_t); }
+
+size_t mfma_program_bytes() { return sizeof(EncProgram); }
+
+void mfma_upload_program(const Layout& L, void* d_program) {
+    const EncProgram host = build_program(L);
+    NLOD_HIP(hipMemcpy(d_program, &host, sizeof(EncProgram), hipMemcpyHostToDevice));
+}
+
+void mfma_pack_weights(hipStream_t stream, const NetShape& net, const float* weights, void* packed) {
+    const uint32_t total = (uint32_t)packed_halves(net);
+    hipLaunchKernelGGL(pack_kernel, dim3(div_round_up(total, 256)), dim3(256), 0, stream, net.width, net.n_input,
+                       k_padded_of(net), net.n_hidden_layers, weights, static_cast<half_t*>(packed), total);
+    NLOD_LAUNCH_CHECK();
+}
+
+}  // namespace nlod

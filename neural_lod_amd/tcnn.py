@@ -54,9 +54,12 @@ class Context:
         self.n = n
 
     def __del__(self):
-        if getattr(self, "_h", None) is not None and self._h.value:
-            _lib.lib().nlod_ctx_destroy(self._h)
-            self._h = None
+        try:
+            if getattr(self, "_h", None) is not None and self._h.value:
+                _lib.lib().nlod_ctx_destroy(self._h)
+        except Exception:  # interpreter shutdown
+            pass
+        self._h = None
 
 
 class NetworkWithInputEncoding:
@@ -87,9 +90,12 @@ class NetworkWithInputEncoding:
         self._inference_params = None
 
     def __del__(self):
-        if getattr(self, "_h", None) is not None and self._h.value:
-            _lib.lib().nlod_model_destroy(self._h)
-            self._h = None
+        try:
+            if getattr(self, "_h", None) is not None and self._h.value:
+                _lib.lib().nlod_model_destroy(self._h)
+        except Exception:  # interpreter shutdown
+            pass
+        self._h = None
 
     # -- layout / parameters ----------------------------------------------------
     def describe(self) -> dict:
