@@ -27,9 +27,12 @@ sys.path.insert(0, str(ROOT))
 
 N_QUERIES = 1 << 20
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# algorithmic bytes per query of the fused inference: 16 levels x 8 corners x
-# 2 features x 4 B gathered + 6 x 4 B input + 1 x 4 B output
-BYTES_PER_QUERY = 16 * 8 * 2 * 4 + 6 * 4 + 4
+# algorithmic bytes per query of the FP16 encoding pass (the dominant kernel):
+# 16 levels x 8 corners x 2 features x 2 B gathered (FP16 table copy),
+# 6 x 4 B input, 48 FP16 features written
+ENC_BYTES_PER_QUERY = 16 * 8 * 2 * 2 + 6 * 4 + 48 * 2
+# whole step (encode + MLP): + features read back + 1 x 4 B output
+BYTES_PER_QUERY = ENC_BYTES_PER_QUERY + 48 * 2 + 4
 FLOP_PER_QUERY = 2 * (48 * 64 + 3 * 64 * 64 + 64 * 1)
 
 
@@ -75,6 +78,19 @@ def main():
         for _ in range(k):
             m.inference(x, o)
 
+    def time_steps(fn, k):
+        fn()
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(k):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
     results = {}
     for name in ("fp32", "fp16"):
         run(name, args.warmup)
@@ -91,6 +107,11 @@ def main():
         ms = e0.elapsed_time(e1) / args.steps
         results[name] = {"ms": ms, "qps": N_QUERIES / (ms * 1e-3), "wall_s": wall}
         log(f"{name}: {ms:.4f} ms/step  {N_QUERIES / ms / 1e6:.2f} Mq/ms...")
+
+    # the dominant kernel of the FP16 step: the per-level encoding pass, timed alone
+    feat = models["fp16"].encode_fragments(x)
+    enc_ms = time_steps(lambda: models["fp16"].encode_fragments(x, feat), args.steps)
+    log(f"fp16 encoding pass alone: {enc_ms:.4f} ms")
 
     # accuracy vs the FP64 oracle on a sample of the same batch
     n_chk = 1 << 14
@@ -118,7 +139,7 @@ def main():
         log(f"cpu baseline: {cpu['value']:.3e} queries/s on {threads} threads")
 
     fp16 = results["fp16"]
-    achieved = BYTES_PER_QUERY * N_QUERIES / (fp16["ms"] * 1e-3) / 1e9
+    achieved = ENC_BYTES_PER_QUERY * N_QUERIES / (enc_ms * 1e-3) / 1e9
     line = {
         "metric": "encode+MLP queries/s & train steps/s, FP16-MFMA vs FP32 plain, <=2e-3 err",
         "value": fp16["qps"],
@@ -141,8 +162,10 @@ def main():
         "speedup_vs_plain_fp32": results["fp32"]["ms"] / fp16["ms"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fused_inference_kernel<64,2,4>",
-                     "bytes_per_query": BYTES_PER_QUERY, "flop_per_query": FLOP_PER_QUERY},
+                     "kernel": "encode_kernel<2,4,half> (per-level hash-grid gathers)",
+                     "kernel_ms": enc_ms, "bytes_per_query": ENC_BYTES_PER_QUERY,
+                     "mlp_kernel_ms": fp16["ms"] - enc_ms, "mlp_flop_per_query": FLOP_PER_QUERY,
+                     "mlp_tflops": FLOP_PER_QUERY * N_QUERIES / ((fp16["ms"] - enc_ms) * 1e-3) / 1e12},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

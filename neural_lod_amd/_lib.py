@@ -75,6 +75,8 @@ SIGNATURES: dict[str, list] = {
     "nlod_ctx_destroy": [_vp],
     "nlod_model_encode": [_vp, _vp, _u32, _vp, _vp, _i32],
     "nlod_model_encode_backward": [_vp, _vp, _u32, _vp, _vp, _vp, _i32],
+    "nlod_model_feature_bytes": [_vp, _u32, C.POINTER(_u64)],
+    "nlod_model_encode_fragments": [_vp, _vp, _u32, _vp, _vp],
 }
 
 

@@ -25,9 +25,12 @@ struct DevSegment {
     uint64_t param_offset;
     GridLevels lv;
 };
+constexpr int kMaxGroups = 64;  // 8-feature groups of the padded encoding (k_padded <= 512)
 struct EncProgram {
     uint32_t n_segs, n_input_dims, enc_width, k_padded;
     DevSegment segs[kMaxSegs];
+    // bit j of grid_rows[g] set: encoded row 8g + j is a grid feature (written by the per-level pass)
+    uint8_t grid_rows[kMaxGroups];
 };
 
 inline uint32_t k_padded_of(const NetShape& n) { return (n.n_input + 31) / 32 * 32; }
@@ -50,15 +53,56 @@ __host__ __device__ inline uint32_t permuted_unit(uint32_t c, uint32_t g, uint32
     return 32u * c + (j < 4 ? 4u * g + j : 16u + 4u * g + (j - 4u));
 }
 
+// Table entry of F features (FP32 or FP16 storage) as floats.
+template <int F, typename TT>
+__device__ __forceinline__ void load_entry(const TT* __restrict__ p, float* out) {
+    if constexpr (std::is_same<TT, float>::value) {
+        if constexpr (F == 1) {
+            out[0] = p[0];
+        } else if constexpr (F == 2) {
+            const float2 t2 = *reinterpret_cast<const float2*>(p);
+            out[0] = t2.x;
+            out[1] = t2.y;
+        } else {
+#pragma unroll
+            for (int f = 0; f < F; f += 4) {
+                const float4 t4 = *reinterpret_cast<const float4*>(p + f);
+                out[f] = t4.x;
+                out[f + 1] = t4.y;
+                out[f + 2] = t4.z;
+                out[f + 3] = t4.w;
+            }
+        }
+    } else {
+        if constexpr (F == 1) {
+            out[0] = (float)p[0];
+        } else if constexpr (F == 2) {
+            typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+            const h2 t = *reinterpret_cast<const h2*>(p);
+            out[0] = (float)t[0];
+            out[1] = (float)t[1];
+        } else {
+#pragma unroll
+            for (int f = 0; f < F; f += 4) {
+                const half4 t = *reinterpret_cast<const half4*>(p + f);
+                out[f] = (float)t[0];
+                out[f + 1] = (float)t[1];
+                out[f + 2] = (float)t[2];
+                out[f + 3] = (float)t[3];
+            }
+        }
+    }
+}
+
 // One level of a grid for one query: F features interpolated in FP32.
-template <int F>
+template <int F, typename TT = float>
 __device__ __forceinline__ void grid_level(const GridLevels& lv, uint32_t level, float x0, float x1, float x2,
-                                           const float* __restrict__ table, float* v) {
+                                           const TT* __restrict__ table, float* v) {
     const float scale = lv.scale[level];
     const uint32_t res = lv.res[level];
     const uint32_t off = lv.offset[level];
     const uint32_t hs = lv.offset[level + 1] - off;
-    const float* tab = table + (size_t)off * F;
+    const TT* tab = table + (size_t)off * F;
     float fr[3];
     uint32_t cell[3];
     grid_pos(x0, scale, lv.interp, fr[0], cell[0]);
@@ -68,8 +112,7 @@ __device__ __forceinline__ void grid_level(const GridLevels& lv, uint32_t level,
     for (int f = 0; f < F; ++f) v[f] = 0.0f;
     if (lv.interp == INTERP_NEAREST) {
         const uint32_t idx = grid_index(lv.grid_type, hs, res, cell[0], cell[1], cell[2]);
-#pragma unroll
-        for (int f = 0; f < F; ++f) v[f] = tab[(size_t)idx * F + f];
+        load_entry<F, TT>(tab + (size_t)idx * F, v);
         return;
     }
     float vals[8][F];
@@ -77,22 +120,7 @@ __device__ __forceinline__ void grid_level(const GridLevels& lv, uint32_t level,
     for (int c = 0; c < 8; ++c) {
         const uint32_t idx = grid_index(lv.grid_type, hs, res, cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
                                         cell[2] + ((c >> 2) & 1));
-        if constexpr (F == 2) {
-            const float2 t2 = *reinterpret_cast<const float2*>(tab + (size_t)idx * 2);
-            vals[c][0] = t2.x;
-            vals[c][1] = t2.y;
-        } else if constexpr (F >= 4) {
-#pragma unroll
-            for (int f = 0; f < F; f += 4) {
-                const float4 t4 = *reinterpret_cast<const float4*>(tab + (size_t)idx * F + f);
-                vals[c][f] = t4.x;
-                vals[c][f + 1] = t4.y;
-                vals[c][f + 2] = t4.z;
-                vals[c][f + 3] = t4.w;
-            }
-        } else {
-            vals[c][0] = tab[idx];
-        }
+        load_entry<F, TT>(tab + (size_t)idx * F, vals[c]);
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
@@ -110,9 +138,10 @@ __device__ __forceinline__ void place(float* win, int j, float v) {
     for (int jj = 0; jj < 8; ++jj) win[jj] = (jj == j) ? v : win[jj];
 }
 
-// Encoded features [w0, w0 + 8) of one query in FP32 (0 beyond the encoded width).
-template <int F, int MAXD>
-__device__ __forceinline__ void encode_window_f32(const EncProgram& P, const float* __restrict__ params,
+// Encoded features [w0, w0 + 8) of one query in FP32 (0 beyond the encoded width;
+// grid rows left at 0 when SKIP_GRIDS).
+template <int F, int MAXD, typename TT = float, bool SKIP_GRIDS = false>
+__device__ __forceinline__ void encode_window_f32(const EncProgram& P, const TT* __restrict__ params,
                                                   const float* __restrict__ q, uint32_t w0, float* win) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) win[j] = 0.0f;
@@ -123,18 +152,19 @@ __device__ __forceinline__ void encode_window_f32(const EncProgram& P, const flo
         if (r1 <= w0 || r0 >= w1) continue;
         const float* x = q + s.in_offset;
         if (s.type == SEG_GRID) {
+            if (SKIP_GRIDS) continue;
             // a grid segment starts at a multiple of F, so levels tile the window
             const int l_begin = w0 > r0 ? (int)((w0 - r0) / F) : 0;
             const int n_levels = (int)s.lv.n_levels;
             const float x0 = x[0], x1 = x[1], x2 = x[2];
-            const float* table = params + s.param_offset;
+            const TT* table = params + s.param_offset;
 #pragma unroll
             for (int t = 0; t < 8 / F; ++t) {
                 const int l = l_begin + t;
                 const int row = (int)r0 + l * F - (int)w0;
                 // gather unconditionally (clamped level: indices stay inside it), place if valid
                 float v[F];
-                grid_level<F>(s.lv, (uint32_t)min(l, n_levels - 1), x0, x1, x2, table, v);
+                grid_level<F, TT>(s.lv, (uint32_t)min(l, n_levels - 1), x0, x1, x2, table, v);
                 const bool ok = l < n_levels && row >= 0 && row < 8;
 #pragma unroll
                 for (int f = 0; f < F; ++f) place(win, ok ? row + f : -1, v[f]);
@@ -170,11 +200,11 @@ __device__ __forceinline__ void encode_window_f32(const EncProgram& P, const flo
     }
 }
 
-template <int F, int MAXD>
-__device__ __forceinline__ half8 encode_window(const EncProgram& P, const float* __restrict__ params,
+template <int F, int MAXD, typename TT = float>
+__device__ __forceinline__ half8 encode_window(const EncProgram& P, const TT* __restrict__ params,
                                                const float* __restrict__ q, uint32_t w0) {
     float win[8];
-    encode_window_f32<F, MAXD>(P, params, q, w0, win);
+    encode_window_f32<F, MAXD, TT>(P, params, q, w0, win);
     half8 h;
 #pragma unroll
     for (int j = 0; j < 8; ++j) h[j] = (half_t)win[j];

@@ -53,6 +53,8 @@ EncProgram build_program(const Layout& L) {
         d.id_offset = s.id_offset;
         d.param_offset = s.param_offset;
         d.lv = s.levels;
+        if (s.type == SEG_GRID)
+            for (uint32_t r = s.out_offset; r < s.out_offset + s.n_out; ++r) P.grid_rows[r / 8] |= (uint8_t)(1u << (r % 8));
     }
     return P;
 }

@@ -1,15 +1,21 @@
 // FP16-MFMA form of the encode + fully fused MLP (gfx950).
 //
-// Fused inference kernel: every wave independently takes tiles of 16 queries.
-// The encoded features never touch memory: for K-chunk c of the first layer,
-// lane (n = lane % 16, g = lane / 16) computes exactly the 8 features
-// 32c + 8g .. 32c + 8g + 7 of query n that v_mfma_f32_16x16x32_f16 wants in
-// its B operand (hash-grid levels gathered and interpolated in FP32, then
-// rounded to FP16).  Hidden activations also stay in registers: the 16x16
-// accumulator layout (4 consecutive unit rows per lane group) is fed straight
-// back as the next layer's B operand, with that layer's weight columns
-// pre-permuted at pack time.  Weights (FP16 A fragments) live in LDS, one copy
-// per workgroup.
+// Two launches per batch:
+//  1. Encoding.  One thread per (query, grid level), blockIdx.y = level, so
+//     the workgroups resident at any time all gather from one level's table:
+//     that level (2 MiB as FP16 at 2^19 entries x 2 features) stays in each
+//     XCD's 4 MiB L2 while 2^20 queries stream through it (the reference's
+//     "one level per block" idea, grid.h:1053-1058).  The last blockIdx.y row
+//     writes the non-grid features (spherical harmonics, identity, padding).
+//     Features are stored FP16 in the MFMA-fragment layout
+//     [N/16][K/8][16 queries][8 features], so phase 2 reads each B fragment
+//     with one coalesced 16-byte load per lane.
+//  2. MLP.  Every wave takes tiles of 16 queries; all layers run on
+//     v_mfma_f32_16x16x32_f16 with FP32 accumulation, weights resident in
+//     LDS, activations never leaving registers: the 16x16 accumulator layout
+//     (4 consecutive unit rows per lane group) is fed straight back as the next
+//     layer's B operand, with that layer's weight columns pre-permuted at pack
+//     time.
 //
 // MFMA operand layout (v_mfma_f32_16x16x32_f16, wave64, g = lane / 16):
 //   A: lane holds A[m = lane % 16][k-slot 8g .. 8g+7]
@@ -29,39 +35,102 @@
 namespace nlod {
 namespace {
 
-constexpr int kBlock = 256;  // lanes per workgroup (4 waves)
+constexpr int kBlock = 256;
+constexpr int kMaxLevelRows = 64;  // grid levels over all grid segments
 
-struct FusedArgs {
-    uint32_t N, n_input_dims, n_output, n_hidden_layers, k_padded;
+struct LevelMap {
+    uint32_t n;                    // grid levels (blockIdx.y rows before the non-grid row)
+    uint8_t seg[kMaxLevelRows];
+    uint8_t level[kMaxLevelRows];
+};
+
+__device__ __forceinline__ size_t feat_index(uint32_t q, uint32_t f, uint32_t kp8) {
+    return (((size_t)(q >> 4) * kp8 + (f >> 3)) * 16 + (q & 15)) * 8 + (f & 7);
+}
+
+template <int F, int MAXD, typename TT>
+__global__ __launch_bounds__(kBlock) void encode_kernel(uint32_t N, uint32_t N16, LevelMap map,
+                                                        const EncProgram* __restrict__ prog,
+                                                        const TT* __restrict__ table, const float* __restrict__ in,
+                                                        half_t* __restrict__ feat) {
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= N16) return;
+    const EncProgram& P = *prog;
+    const uint32_t kp8 = P.k_padded / 8;
+    const bool valid = q < N;
+    const float* x = in + (size_t)(valid ? q : 0) * P.n_input_dims;
+    if (blockIdx.y < map.n) {
+        const DevSegment& s = P.segs[map.seg[blockIdx.y]];
+        const uint32_t level = map.level[blockIdx.y];
+        const float* xs = x + s.in_offset;
+        float v[F];
+        grid_level<F, TT>(s.lv, level, xs[0], xs[1], xs[2], table + s.param_offset, v);
+        const uint32_t f0 = s.out_offset + level * F;  // multiple of F: the F halves stay in one group
+        half_t h[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) h[f] = valid ? (half_t)v[f] : (half_t)0.0f;
+        half_t* dst = feat + feat_index(q, f0, kp8);
+        if constexpr (F == 1) {
+            dst[0] = h[0];
+        } else if constexpr (F == 2) {
+            typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+            *reinterpret_cast<h2*>(dst) = h2{h[0], h[1]};
+        } else if constexpr (F == 4) {
+            *reinterpret_cast<half4*>(dst) = half4{h[0], h[1], h[2], h[3]};
+        } else {
+            half8 o;
+#pragma unroll
+            for (int f = 0; f < 8; ++f) o[f] = h[f];
+            *reinterpret_cast<half8*>(dst) = o;
+        }
+        return;
+    }
+    // non-grid rows: SH, identity, padding (grid pad rows and the K tail are 0)
+    for (uint32_t g8 = 0; g8 < kp8; ++g8) {
+        const uint32_t mask = P.grid_rows[g8];
+        if (mask == 0xffu) continue;
+        float win[8];
+        encode_window_f32<F, MAXD, TT, true>(P, table, x, 8 * g8, win);
+        half_t* dst = feat + feat_index(q, 8 * g8, kp8);
+        if (mask == 0) {
+            half8 o;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) o[j] = valid ? (half_t)win[j] : (half_t)0.0f;
+            *reinterpret_cast<half8*>(dst) = o;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (!((mask >> j) & 1u)) dst[j] = valid ? (half_t)win[j] : (half_t)0.0f;
+        }
+    }
+}
+
+struct MlpArgs16 {
+    uint32_t N, n_output, n_hidden_layers, k_padded;
     int32_t act, out_act;
     uint32_t packed_halves;
 };
 
-template <int W, int F, int MAXD>
-__global__ __launch_bounds__(kBlock) void fused_inference_kernel(FusedArgs a, const EncProgram* __restrict__ prog,
-                                                                 const float* __restrict__ params,
-                                                                 const half_t* __restrict__ packed,
-                                                                 const float* __restrict__ in,
-                                                                 float* __restrict__ out) {
+template <int W>
+__global__ __launch_bounds__(kBlock) void mlp_fwd_kernel(MlpArgs16 a, const half_t* __restrict__ packed,
+                                                         const half_t* __restrict__ feat, float* __restrict__ out) {
     constexpr int MT = W / 16;
     extern __shared__ float4 smem4[];
     half_t* sw = reinterpret_cast<half_t*>(smem4);
     for (uint32_t k = threadIdx.x; k < a.packed_halves / 8; k += kBlock)
         reinterpret_cast<float4*>(sw)[k] = reinterpret_cast<const float4*>(packed)[k];
     __syncthreads();
-    const EncProgram& P = *prog;
     const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
-    const uint32_t kc0 = a.k_padded / 32;
+    const uint32_t kc0 = a.k_padded / 32, kp8 = a.k_padded / 8;
     const uint32_t n_tiles = (a.N + 15) / 16;
     const uint32_t n_waves = gridDim.x * (kBlock / 64);
     for (uint32_t tile = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); tile < n_tiles; tile += n_waves) {
-        const uint32_t q = tile * 16 + n;
-        const float* qin = in + (size_t)min(q, a.N - 1) * a.n_input_dims;
         floatx4 acc[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) acc[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+        const half_t* fb = feat + (size_t)tile * kp8 * 128;
         for (uint32_t c = 0; c < kc0; ++c) {
-            const half8 b = encode_window<F, MAXD>(P, params, qin, 32 * c + 8 * g);
+            const half8 b = *reinterpret_cast<const half8*>(fb + ((size_t)(4 * c + g) * 16 + n) * 8);
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 const half8 w = *reinterpret_cast<const half8*>(sw + ((size_t)(mt * kc0 + c) * 64 + lane) * 8);
@@ -69,6 +138,7 @@ __global__ __launch_bounds__(kBlock) void fused_inference_kernel(FusedArgs a, co
             }
         }
         const floatx4 o = mlp_tail16<W>(sw + (size_t)MT * kc0 * 512, acc, a.n_hidden_layers, a.act);
+        const uint32_t q = tile * 16 + n;
         if (q < a.N) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -79,21 +149,32 @@ __global__ __launch_bounds__(kBlock) void fused_inference_kernel(FusedArgs a, co
     }
 }
 
-template <int W, int F, int MAXD>
-void launch_fused(hipStream_t stream, const FusedArgs& a, const EncProgram* prog, const float* params,
-                  const half_t* packed, const float* in, float* out, size_t smem) {
-    auto kern = fused_inference_kernel<W, F, MAXD>;
-    static int occupancy = -1;
-    if (occupancy < 0) {
-        NLOD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-        int blocks_per_cu = 0;
-        NLOD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kern, kBlock, smem));
-        occupancy = std::max(1, blocks_per_cu);
+__global__ void to_half_kernel(const float* __restrict__ src, half_t* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = (half_t)src[i];
+}
+
+LevelMap make_level_map(const Layout& L) {
+    LevelMap m{};
+    for (size_t si = 0; si < L.segs.size(); ++si) {
+        const Segment& s = L.segs[si];
+        if (s.type != SEG_GRID) continue;
+        for (uint32_t l = 0; l < s.n_levels; ++l) {
+            if (m.n >= (uint32_t)kMaxLevelRows) fail(NLOD_ERR_UNSUPPORTED, "more than 64 grid levels in one model");
+            m.seg[m.n] = (uint8_t)si;
+            m.level[m.n] = (uint8_t)l;
+            ++m.n;
+        }
     }
-    const uint32_t n_tiles = div_round_up(a.N, 16);
-    const uint32_t blocks = std::min<uint32_t>(div_round_up(n_tiles, kBlock / 64), (uint32_t)(cu_count() * occupancy));
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), smem, stream, a, prog, params, packed, in, out);
-    NLOD_LAUNCH_CHECK();
+    return m;
+}
+
+template <typename Kern>
+int occupancy_of(Kern kern, size_t smem) {
+    int b = 0;
+    NLOD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    NLOD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, kBlock, smem));
+    return std::max(b, 1);
 }
 
 }  // namespace
@@ -104,22 +185,49 @@ bool mfma_supported(const Layout& L) {
     if (L.net.width != 32 && L.net.width != 64 && L.net.width != 128) return false;
     if (L.segs.size() > (size_t)kMaxSegs) return false;
     if (L.net.n_output > 16) return false;
+    if (k_padded_of(L.net) > 8 * kMaxGroups) return false;
     if (mfma_fused_smem_bytes(L.net) > 160 * 1024) return false;
-    uint32_t F = 0;
+    uint32_t F = 0, levels = 0;
     for (const Segment& s : L.segs) {
         if (s.type != SEG_GRID) continue;
-        if (F && s.F != F) return false;  // one feature count per model in the fused kernels
+        if (F && s.F != F) return false;  // one feature count per model
         F = s.F;
+        levels += s.n_levels;
     }
-    return true;
+    return levels <= (uint32_t)kMaxLevelRows;
 }
 
-void mfma_fused_inference(hipStream_t stream, const Layout& L, const void* d_program, const float* params,
-                          const void* packed, const float* in, uint32_t N, float* out) {
+size_t mfma_feature_bytes(const Layout& L, uint32_t N) {
+    return sizeof(half_t) * (size_t)div_round_up(N, 16) * 16 * k_padded_of(L.net);
+}
+
+void mfma_to_half(hipStream_t stream, const float* src, void* dst, size_t n) {
+    if (n == 0) return;
+    const uint32_t blocks = (uint32_t)std::min<size_t>(div_round_up(n, 256), 8192);
+    hipLaunchKernelGGL(to_half_kernel, dim3(blocks), dim3(256), 0, stream, src, static_cast<half_t*>(dst), n);
+    NLOD_LAUNCH_CHECK();
+}
+
+void mfma_encode(hipStream_t stream, const Layout& L, const void* d_program, const void* table16, const float* in,
+                 uint32_t N, void* feat) {
     if (N == 0) return;
-    FusedArgs a{};
+    const LevelMap map = make_level_map(L);
+    const uint32_t N16 = div_round_up(N, 16) * 16;
+    const dim3 grid(div_round_up(N16, kBlock), map.n + 1);
+    dispatch_fused(L, [&](auto Wc, auto Fc, auto Dc) {
+        constexpr int F = decltype(Fc)::value, D = decltype(Dc)::value;
+        hipLaunchKernelGGL((encode_kernel<F, D, half_t>), grid, dim3(kBlock), 0, stream, N, N16, map,
+                           static_cast<const EncProgram*>(d_program), static_cast<const half_t*>(table16), in,
+                           static_cast<half_t*>(feat));
+    });
+    NLOD_LAUNCH_CHECK();
+}
+
+void mfma_mlp_forward(hipStream_t stream, const Layout& L, const void* packed, const void* feat, uint32_t N,
+                      float* out) {
+    if (N == 0) return;
+    MlpArgs16 a{};
     a.N = N;
-    a.n_input_dims = L.n_input_dims;
     a.n_output = L.net.n_output;
     a.n_hidden_layers = L.net.n_hidden_layers;
     a.k_padded = k_padded_of(L.net);
@@ -127,12 +235,21 @@ void mfma_fused_inference(hipStream_t stream, const Layout& L, const void* d_pro
     a.out_act = L.net.output_activation;
     a.packed_halves = (uint32_t)packed_halves(L.net);
     const size_t smem = mfma_fused_smem_bytes(L.net);
-    const EncProgram* prog = static_cast<const EncProgram*>(d_program);
-    const half_t* pk = static_cast<const half_t*>(packed);
-    dispatch_fused(L, [&](auto Wc, auto Fc, auto Dc) {
-        launch_fused<decltype(Wc)::value, decltype(Fc)::value, decltype(Dc)::value>(stream, a, prog, params, pk, in,
-                                                                                    out, smem);
-    });
+    const uint32_t n_tiles = div_round_up(N, 16);
+    auto launch = [&](auto kern) {
+        static int occ = -1;
+        if (occ < 0) occ = occupancy_of(kern, smem);
+        const uint32_t blocks = std::min<uint32_t>(div_round_up(n_tiles, kBlock / 64), (uint32_t)(cu_count() * occ));
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), smem, stream, a, static_cast<const half_t*>(packed),
+                           static_cast<const half_t*>(feat), out);
+    };
+    switch (L.net.width) {
+        case 32: launch(mlp_fwd_kernel<32>); break;
+        case 64: launch(mlp_fwd_kernel<64>); break;
+        case 128: launch(mlp_fwd_kernel<128>); break;
+        default: fail(NLOD_ERR_UNSUPPORTED, "FP16 MFMA kernels support widths 32, 64 and 128");
+    }
+    NLOD_LAUNCH_CHECK();
 }
 
 }  // namespace nlod

@@ -2,6 +2,7 @@
 // and its C ABI: parameters are bound by pointer (network first, then the
 // encoding, network_with_input_encoding.h:113-128); inference, forward with a
 // saved context, and backward producing parameter / input gradients.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -32,6 +33,10 @@ void Model::set_params(hipStream_t s, const float* p, const float* p_inf) {
         packed_inference.ensure(bytes);
         mfma_pack_weights(s, L.net, params, packed.p);
         mfma_pack_weights(s, L.net, params_inference, packed_inference.p);
+        table16.ensure(sizeof(uint16_t) * L.n_params);
+        table16_inference.ensure(sizeof(uint16_t) * L.n_params);
+        mfma_to_half(s, params, table16.p, L.n_params);
+        mfma_to_half(s, params_inference, table16_inference.p, L.n_params);
     }
 }
 
@@ -49,7 +54,14 @@ void Model::inference(hipStream_t s, uint32_t N, const float* in, float* out) {
     require_params(true);
     if (N == 0) return;
     if (precision == PREC_FP16) {
-        mfma_fused_inference(s, L, program.p, params_inference, packed_inference.p, in, N, out);
+        // chunks of at most 2^21 queries bound the feature workspace
+        const uint32_t chunk = 1u << 21;
+        ws_feat.ensure(mfma_feature_bytes(L, std::min(N, chunk)));
+        for (uint32_t q0 = 0; q0 < N; q0 += chunk) {
+            const uint32_t n = std::min(chunk, N - q0);
+            mfma_encode(s, L, program.p, table16_inference.p, in + (size_t)q0 * L.n_input_dims, n, ws_feat.p);
+            mfma_mlp_forward(s, L, packed_inference.p, ws_feat.p, n, out + (size_t)q0 * L.net.n_output);
+        }
         return;
     }
     ws_enc.ensure(sizeof(float) * (size_t)L.enc_width * N);
@@ -240,6 +252,24 @@ int nlod_model_encode_backward(nlod_model* model, void* stream, uint32_t n, cons
         hipStream_t s = nlod::as_stream(stream);
         if (grad_mode == nlod::GRAD_OVERWRITE) nlod::fill_f32(s, d_dL_dparams, m.L.n_params, 0.0f);
         if (n) nlod::encode_backward_params(s, m.L, d_in, n, d_dL_dencoded, d_dL_dparams);
+    });
+}
+
+int nlod_model_feature_bytes(const nlod_model* model, uint32_t n, uint64_t* bytes_out) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(model && bytes_out, "null argument");
+        if (model->m->precision != nlod::PREC_FP16) nlod::fail(NLOD_ERR_UNSUPPORTED, "FP16 models only");
+        *bytes_out = nlod::mfma_feature_bytes(model->m->L, n);
+    });
+}
+
+int nlod_model_encode_fragments(nlod_model* model, void* stream, uint32_t n, const float* d_in, void* d_features) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(model && (n == 0 || (d_in && d_features)), "null argument");
+        nlod::Model& m = *model->m;
+        if (m.precision != nlod::PREC_FP16) nlod::fail(NLOD_ERR_UNSUPPORTED, "FP16 models only");
+        m.require_params(true);
+        nlod::mfma_encode(nlod::as_stream(stream), m.L, m.program.p, m.table16_inference.p, d_in, n, d_features);
     });
 }
 

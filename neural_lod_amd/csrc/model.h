@@ -61,6 +61,9 @@ struct Model {
     // FP16 MFMA weights (network part), packed for the fused kernels, and the
     // device copy of the encoding description they read
     DevBuf packed, packed_inference, program;
+    // FP16 copies of the parameter vectors (the grid tables the encoding pass gathers)
+    DevBuf table16, table16_inference;
+    DevBuf ws_feat;
     // scratch
     DevBuf ws_enc, ws_delta, ws_delta_out, ws_denc, ws_wgrad;
 
@@ -87,8 +90,15 @@ size_t mfma_fused_smem_bytes(const NetShape& net);
 // encoding description for the fused kernels, copied to device memory (synchronous)
 void mfma_upload_program(const Layout& L, void* d_program);
 void mfma_pack_weights(hipStream_t stream, const NetShape& net, const float* weights, void* packed);
-// fused encode + MLP inference
-void mfma_fused_inference(hipStream_t stream, const Layout& L, const void* d_program, const float* params,
-                          const void* packed, const float* in, uint32_t N, float* out);
+// FP32 -> FP16 copy (the grid tables read by the encoding pass)
+void mfma_to_half(hipStream_t stream, const float* src, void* dst, size_t n);
+// bytes of the FP16 feature buffer for N queries ([N/16][K/8][16][8] halves)
+size_t mfma_feature_bytes(const Layout& L, uint32_t N);
+// encoding pass (per-level grid gathers + non-grid rows) -> FP16 feature fragments
+void mfma_encode(hipStream_t stream, const Layout& L, const void* d_program, const void* table16, const float* in,
+                 uint32_t N, void* feat);
+// MLP on MFMA from the feature fragments -> out float[N][n_output]
+void mfma_mlp_forward(hipStream_t stream, const Layout& L, const void* packed, const void* feat, uint32_t N,
+                      float* out);
 
 }  // namespace nlod

@@ -131,6 +131,19 @@ class NetworkWithInputEncoding:
         _lib.call("nlod_model_inference", self._h, _stream(x.device), n, _ptr(x), _ptr(out))
         return out
 
+    def encode_fragments(self, x: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """FP16 models: encoding pass alone into the MFMA fragment layout (uint8 buffer)."""
+        _check_f32(x, "input")
+        n = x.shape[0]
+        nb = C.c_uint64()
+        _lib.call("nlod_model_feature_bytes", self._h, n, C.byref(nb))
+        if out is None:
+            out = torch.empty(nb.value, dtype=torch.uint8, device=x.device)
+        elif out.numel() * out.element_size() < nb.value:
+            raise ValueError(f"feature buffer needs {nb.value} bytes")
+        _lib.call("nlod_model_encode_fragments", self._h, _stream(x.device), n, _ptr(x), _ptr(out))
+        return out
+
     def encode(self, x: torch.Tensor, use_inference_params: bool = True) -> torch.Tensor:
         """Encoded features as [encoded_width, n] (SoA)."""
         _check_f32(x, "input")
