@@ -259,7 +259,6 @@ __device__ __forceinline__ floatx4 mlp_tail16(const half_t* __restrict__ frag, f
     return o;
 }
 
-int cu_count();
 
 // Calls f(W, F, MAXD) with integral constants matching the layout.
 template <typename Fn>

@@ -2,6 +2,10 @@
 // MFMA kernels (layouts documented in mfma_common.h).
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "mfma_common.h"
 #include "model.h"
 
@@ -69,6 +73,23 @@ int cu_count() {
         return c;
     }();
     return n;
+}
+
+int occupancy_blocks(const void* kernel, int block, size_t smem) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, const void*, int, size_t>, int> cache;
+    int dev = 0;
+    NLOD_HIP(hipGetDevice(&dev));
+    const auto key = std::make_tuple(dev, kernel, block, smem);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    if (smem) NLOD_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    int b = 0;
+    NLOD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kernel, block, smem));
+    b = b > 0 ? b : 1;
+    cache.emplace(key, b);
+    return b;
 }
 
 size_t mfma_packed_bytes(const NetShape& net) { return packed_halves(net) * sizeof(half_t); }

@@ -339,19 +339,9 @@ size_t smem_bytes() {
 
 template <int W, typename Kern>
 uint32_t grid_for(Kern kern, uint32_t N) {
-    static int occ = -1;
-    if (occ < 0) {
-        NLOD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)smem_bytes<W>()));
-        int b = 0;
-        NLOD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, kThreads, smem_bytes<W>()));
-        occ = std::max(b, 1);
-    }
-    int dev = 0, cus = 0;
-    NLOD_HIP(hipGetDevice(&dev));
-    NLOD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int occ = occupancy_blocks((const void*)kern, kThreads, smem_bytes<W>());
     const uint32_t tiles = div_round_up(N, Tiling<W>::T);
-    return std::min<uint32_t>(tiles, (uint32_t)(cus * occ));
+    return std::min<uint32_t>(tiles, (uint32_t)(cu_count() * occ));
 }
 
 void wgrad_one(hipStream_t stream, uint32_t N, uint32_t M, uint32_t Kb, const float* A, const float* B, float* out,

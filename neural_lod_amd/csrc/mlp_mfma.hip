@@ -169,14 +169,6 @@ LevelMap make_level_map(const Layout& L) {
     return m;
 }
 
-template <typename Kern>
-int occupancy_of(Kern kern, size_t smem) {
-    int b = 0;
-    NLOD_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-    NLOD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, kern, kBlock, smem));
-    return std::max(b, 1);
-}
-
 }  // namespace
 
 size_t mfma_fused_smem_bytes(const NetShape& net) { return mfma_packed_bytes(net); }
@@ -237,8 +229,7 @@ void mfma_mlp_forward(hipStream_t stream, const Layout& L, const void* packed, c
     const size_t smem = mfma_fused_smem_bytes(L.net);
     const uint32_t n_tiles = div_round_up(N, 16);
     auto launch = [&](auto kern) {
-        static int occ = -1;
-        if (occ < 0) occ = occupancy_of(kern, smem);
+        const int occ = occupancy_blocks((const void*)kern, kBlock, smem);
         const uint32_t blocks = std::min<uint32_t>(div_round_up(n_tiles, kBlock / 64), (uint32_t)(cu_count() * occ));
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), smem, stream, a, static_cast<const half_t*>(packed),
                            static_cast<const half_t*>(feat), out);

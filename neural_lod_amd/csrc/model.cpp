@@ -28,11 +28,15 @@ void Model::set_params(hipStream_t s, const float* p, const float* p_inf) {
     params = p;
     params_inference = p_inf ? p_inf : p;
     if (precision == PREC_FP16) {
-        const size_t bytes = mfma_packed_bytes(L.net);
+        const size_t bytes = mfma_packed_bytes(L.net), bwd_bytes = mfma_packed_bwd_bytes(L.net);
         packed.ensure(bytes);
         packed_inference.ensure(bytes);
+        packed_bwd.ensure(bwd_bytes);
+        packed_bwd_inference.ensure(bwd_bytes);
         mfma_pack_weights(s, L.net, params, packed.p);
         mfma_pack_weights(s, L.net, params_inference, packed_inference.p);
+        mfma_pack_bwd_weights(s, L.net, params, packed_bwd.p);
+        mfma_pack_bwd_weights(s, L.net, params_inference, packed_bwd_inference.p);
         table16.ensure(sizeof(uint16_t) * L.n_params);
         table16_inference.ensure(sizeof(uint16_t) * L.n_params);
         mfma_to_half(s, params, table16.p, L.n_params);
@@ -50,45 +54,107 @@ void Model::encode(hipStream_t s, uint32_t N, const float* in, float* enc_soa, b
     encode_forward_f32(s, L, weights(inference), in, N, enc_soa);
 }
 
-void Model::inference(hipStream_t s, uint32_t N, const float* in, float* out) {
-    require_params(true);
+void Model::forward_outputs(hipStream_t s, uint32_t N, const float* in, float* out, bool use_inference) {
     if (N == 0) return;
+    // chunks bound the feature / encoding workspace
+    const uint32_t chunk = kRecomputeAbove;
     if (precision == PREC_FP16) {
-        // chunks of at most 2^21 queries bound the feature workspace
-        const uint32_t chunk = 1u << 21;
         ws_feat.ensure(mfma_feature_bytes(L, std::min(N, chunk)));
         for (uint32_t q0 = 0; q0 < N; q0 += chunk) {
             const uint32_t n = std::min(chunk, N - q0);
-            mfma_encode(s, L, program.p, table16_inference.p, in + (size_t)q0 * L.n_input_dims, n, ws_feat.p);
-            mfma_mlp_forward(s, L, packed_inference.p, ws_feat.p, n, out + (size_t)q0 * L.net.n_output);
+            mfma_encode(s, L, program.p, table_half(use_inference), in + (size_t)q0 * L.n_input_dims, n, ws_feat.p);
+            mfma_mlp_forward(s, L, packed_fwd(use_inference), ws_feat.p, n, out + (size_t)q0 * L.net.n_output);
         }
         return;
     }
-    ws_enc.ensure(sizeof(float) * (size_t)L.enc_width * N);
-    encode_forward_f32(s, L, params_inference, in, N, ws_enc.as<float>());
-    mlp_forward_f32(s, L.net, params_inference, ws_enc.as<float>(), N, out, nullptr);
+    ws_enc.ensure(sizeof(float) * (size_t)L.enc_width * std::min(N, chunk));
+    for (uint32_t q0 = 0; q0 < N; q0 += chunk) {
+        const uint32_t n = std::min(chunk, N - q0);
+        encode_forward_f32(s, L, weights(use_inference), in + (size_t)q0 * L.n_input_dims, n, ws_enc.as<float>());
+        mlp_forward_f32(s, L.net, weights(use_inference), ws_enc.as<float>(), n, out + (size_t)q0 * L.net.n_output,
+                        nullptr);
+    }
+}
+
+void Model::inference(hipStream_t s, uint32_t N, const float* in, float* out) {
+    require_params(true);
+    forward_outputs(s, N, in, out, true);
+}
+
+static size_t enc_bytes(const Layout& L, int precision, uint32_t N) {
+    return precision == PREC_FP16 ? mfma_feature_bytes(L, N) : sizeof(float) * (size_t)L.enc_width * N;
+}
+static size_t hidden_bytes(const Layout& L, int precision, uint32_t N) {
+    return precision == PREC_FP16 ? mfma_hidden_bytes(L, N)
+                                  : sizeof(float) * (size_t)L.net.n_hidden_layers * L.net.width * N;
+}
+
+void Model::forward_saved(hipStream_t s, uint32_t N, const float* in, bool use_inference, void* enc, void* hidden,
+                          float* out) {
+    if (precision == PREC_FP16) {
+        mfma_encode(s, L, program.p, table_half(use_inference), in, N, enc);
+        mfma_mlp_forward_train(s, L, packed_fwd(use_inference), enc, N, hidden, out);
+        return;
+    }
+    encode_forward_f32(s, L, weights(use_inference), in, N, static_cast<float*>(enc));
+    mlp_forward_f32(s, L.net, weights(use_inference), static_cast<const float*>(enc), N, out,
+                    static_cast<float*>(hidden));
 }
 
 std::unique_ptr<ForwardCtx> Model::forward(hipStream_t s, uint32_t N, const float* in, float* out, bool use_inference,
                                            bool prepare_input_gradients) {
     require_params(use_inference);
-    if (precision == PREC_FP16)
-        fail(NLOD_ERR_UNSUPPORTED, "FP16 forward with saved context is not available in this build");
     auto ctx = std::make_unique<ForwardCtx>();
     ctx->N = N;
     ctx->use_inference_params = use_inference;
     ctx->prepare_input_gradients = prepare_input_gradients;
     if (N == 0) return ctx;
-    const NetShape& n = L.net;
-    ctx->enc.ensure(sizeof(float) * (size_t)L.enc_width * N);
-    ctx->hidden.ensure(sizeof(float) * (size_t)n.n_hidden_layers * n.width * N);
-    ctx->out.ensure(sizeof(float) * (size_t)n.n_output * N);
-    const float* w = weights(use_inference);
-    encode_forward_f32(s, L, w, in, N, ctx->enc.as<float>());
-    mlp_forward_f32(s, n, w, ctx->enc.as<float>(), N, ctx->out.as<float>(), ctx->hidden.as<float>());
-    if (out)
-        NLOD_HIP(hipMemcpyAsync(out, ctx->out.p, sizeof(float) * (size_t)n.n_output * N, hipMemcpyDeviceToDevice, s));
+    const size_t out_bytes = sizeof(float) * (size_t)L.net.n_output * N;
+    ctx->out.ensure(out_bytes);
+    ctx->recompute = N > kRecomputeAbove;
+    if (ctx->recompute) {
+        // activations are recomputed chunk by chunk in backward
+        forward_outputs(s, N, in, ctx->out.as<float>(), use_inference);
+    } else {
+        ctx->enc.ensure(enc_bytes(L, precision, N));
+        ctx->hidden.ensure(hidden_bytes(L, precision, N));
+        forward_saved(s, N, in, use_inference, ctx->enc.p, ctx->hidden.p, ctx->out.as<float>());
+    }
+    if (out) NLOD_HIP(hipMemcpyAsync(out, ctx->out.p, out_bytes, hipMemcpyDeviceToDevice, s));
     return ctx;
+}
+
+void Model::backward_saved(hipStream_t s, uint32_t N, bool use_inference, const float* in, const float* d_out,
+                           const void* enc, const void* hidden, const float* out, float* d_params, float* d_in,
+                           bool param_grads, bool net_accumulate) {
+    const NetShape& n = L.net;
+    const bool grid_grads = param_grads && L.n_params > L.n_net_params;
+    const bool need_denc = grid_grads || d_in;
+    if (need_denc) ws_denc.ensure(sizeof(float) * (size_t)L.enc_width * N);
+    float* d_enc = need_denc ? ws_denc.as<float>() : nullptr;
+    if (precision == PREC_FP16) {
+        ws_delta.ensure(mfma_delta_bytes(L, N));
+        mfma_mlp_backward(s, L, packed_back(use_inference), hidden, out, d_out, N, ws_delta.p, d_enc);
+        if (param_grads) {
+            const size_t wsf = mfma_wgrad_workspace_floats(L, N);
+            ws_wgrad.ensure(sizeof(float) * wsf);
+            mfma_weight_grad(s, L, enc, hidden, ws_delta.p, N, d_params, ws_wgrad.as<float>(), wsf, net_accumulate);
+        }
+    } else {
+        const float* w = weights(use_inference);
+        const float* h = static_cast<const float*>(hidden);
+        ws_delta.ensure(sizeof(float) * (size_t)n.n_hidden_layers * n.width * N);
+        ws_delta_out.ensure(sizeof(float) * (size_t)n.n_output_padded * N);
+        mlp_backward_delta_f32(s, n, w, h, out, d_out, N, ws_delta.as<float>(), ws_delta_out.as<float>(), d_enc);
+        if (param_grads) {
+            const size_t wsf = mlp_weight_grad_workspace_floats(n, N);
+            ws_wgrad.ensure(sizeof(float) * wsf);
+            mlp_weight_grad_f32(s, n, static_cast<const float*>(enc), h, ws_delta.as<float>(),
+                                ws_delta_out.as<float>(), N, d_params, ws_wgrad.as<float>(), wsf, net_accumulate);
+        }
+    }
+    if (grid_grads) encode_backward_params(s, L, in, N, d_enc, d_params);
+    if (d_in) encode_backward_input(s, L, in, N, d_enc, d_in);
 }
 
 void Model::backward(hipStream_t s, ForwardCtx& ctx, const float* in, const float* d_out, float* d_params, float* d_in,
@@ -98,31 +164,34 @@ void Model::backward(hipStream_t s, ForwardCtx& ctx, const float* in, const floa
     if (d_in && !encode_backward_input_supported(L))
         fail(NLOD_ERR_UNSUPPORTED, "input gradients through a grid encoding are not supported");
     const uint32_t N = ctx.N;
-    const NetShape& n = L.net;
+    const bool param_grads = grad_mode != GRAD_IGNORE;
+    if (grad_mode == GRAD_OVERWRITE && L.n_params > L.n_net_params)
+        fill_f32(s, d_params + L.n_net_params, L.n_params - L.n_net_params, 0.0f);
     if (N == 0) {
-        if (grad_mode == GRAD_OVERWRITE) fill_f32(s, d_params, L.n_params, 0.0f);
+        if (grad_mode == GRAD_OVERWRITE) fill_f32(s, d_params, L.n_net_params, 0.0f);
         return;
     }
     NLOD_CHECK(in && d_out, "input / output-gradient pointer is null");
-    const float* w = weights(ctx.use_inference_params);
-    ws_delta.ensure(sizeof(float) * (size_t)n.n_hidden_layers * n.width * N);
-    ws_delta_out.ensure(sizeof(float) * (size_t)n.n_output_padded * N);
-    const bool need_denc = (grad_mode != GRAD_IGNORE && L.n_params > L.n_net_params) || d_in;
-    if (need_denc) ws_denc.ensure(sizeof(float) * (size_t)L.enc_width * N);
-    mlp_backward_delta_f32(s, n, w, ctx.hidden.as<float>(), ctx.out.as<float>(), d_out, N, ws_delta.as<float>(),
-                           ws_delta_out.as<float>(), need_denc ? ws_denc.as<float>() : nullptr);
-    if (grad_mode != GRAD_IGNORE) {
-        const size_t wsf = mlp_weight_grad_workspace_floats(n, N);
-        ws_wgrad.ensure(sizeof(float) * wsf);
-        mlp_weight_grad_f32(s, n, ctx.enc.as<float>(), ctx.hidden.as<float>(), ws_delta.as<float>(),
-                            ws_delta_out.as<float>(), N, d_params, ws_wgrad.as<float>(), wsf,
-                            grad_mode == GRAD_ACCUMULATE);
-        if (L.n_params > L.n_net_params) {
-            if (grad_mode == GRAD_OVERWRITE) fill_f32(s, d_params + L.n_net_params, L.n_params - L.n_net_params, 0.0f);
-            encode_backward_params(s, L, in, N, ws_denc.as<float>(), d_params);
-        }
+    const bool use_inf = ctx.use_inference_params;
+    if (!ctx.recompute) {
+        backward_saved(s, N, use_inf, in, d_out, ctx.enc.p, ctx.hidden.p, ctx.out.as<float>(), d_params, d_in,
+                       param_grads, grad_mode == GRAD_ACCUMULATE);
+        return;
     }
-    if (d_in) encode_backward_input(s, L, in, N, ws_denc.as<float>(), d_in);
+    // recompute the activations of each chunk, then its backward; network
+    // gradients of later chunks are added to the first chunk's
+    const uint32_t chunk = kRecomputeAbove;
+    ws_feat.ensure(enc_bytes(L, precision, chunk));
+    ws_hidden.ensure(hidden_bytes(L, precision, chunk));
+    ws_out.ensure(sizeof(float) * (size_t)L.net.n_output * chunk);
+    for (uint32_t q0 = 0; q0 < N; q0 += chunk) {
+        const uint32_t n = std::min(chunk, N - q0);
+        const float* in_c = in + (size_t)q0 * L.n_input_dims;
+        forward_saved(s, n, in_c, use_inf, ws_feat.p, ws_hidden.p, ws_out.as<float>());
+        backward_saved(s, n, use_inf, in_c, d_out + (size_t)q0 * L.net.n_output, ws_feat.p, ws_hidden.p,
+                       ws_out.as<float>(), d_params, d_in ? d_in + (size_t)q0 * L.n_input_dims : nullptr, param_grads,
+                       grad_mode == GRAD_ACCUMULATE || q0 > 0);
+    }
 }
 
 // Parameter initialisation of tiny-cuda-nn's Trainer (trainer.h:53-92):

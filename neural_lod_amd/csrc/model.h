@@ -61,11 +61,15 @@ struct Model {
     // FP16 MFMA weights (network part), packed for the fused kernels, and the
     // device copy of the encoding description they read
     DevBuf packed, packed_inference, program;
+    // FP16 MFMA transposed weights for the backward products
+    DevBuf packed_bwd, packed_bwd_inference;
     // FP16 copies of the parameter vectors (the grid tables the encoding pass gathers)
     DevBuf table16, table16_inference;
     DevBuf ws_feat;
     // scratch
     DevBuf ws_enc, ws_delta, ws_delta_out, ws_denc, ws_wgrad;
+    // recomputed activations of one chunk (batches above kRecomputeAbove)
+    DevBuf ws_hidden, ws_out;
 
     Model(uint32_t n_in, uint32_t n_out, const Json& enc, const Json& net, int prec);
 
@@ -79,8 +83,29 @@ struct Model {
                                         bool prepare_input_gradients);
     void backward(hipStream_t s, ForwardCtx& ctx, const float* in, const float* d_out, float* d_params,
                   float* d_in, int grad_mode);
+    const void* packed_fwd(bool inference) const { return inference ? packed_inference.p : packed.p; }
+    const void* packed_back(bool inference) const { return inference ? packed_bwd_inference.p : packed_bwd.p; }
+    const void* table_half(bool inference) const { return inference ? table16_inference.p : table16.p; }
+
+  private:
+    // outputs only (no saved activations), chunked
+    void forward_outputs(hipStream_t s, uint32_t N, const float* in, float* out, bool use_inference);
+    // activations of a batch into enc / hidden / out
+    void forward_saved(hipStream_t s, uint32_t N, const float* in, bool use_inference, void* enc, void* hidden,
+                       float* out);
+    // backward of one batch from its saved activations; net_accumulate: add the
+    // network gradients to d_params instead of overwriting them
+    void backward_saved(hipStream_t s, uint32_t N, bool use_inference, const float* in, const float* d_out,
+                        const void* enc, const void* hidden, const float* out, float* d_params, float* d_in,
+                        bool param_grads, bool net_accumulate);
+
+  public:
     void init_params_host(uint32_t seed, float* out) const;
 };
+
+// Batches above this size recompute their activations chunk by chunk in the
+// backward pass instead of storing them in the forward pass.
+constexpr uint32_t kRecomputeAbove = 1u << 20;
 
 // ---- mlp_mfma.hip (FP16 MFMA form) ----------------------------------------------
 bool mfma_supported(const Layout& L);
@@ -100,5 +125,24 @@ void mfma_encode(hipStream_t stream, const Layout& L, const void* d_program, con
 // MLP on MFMA from the feature fragments -> out float[N][n_output]
 void mfma_mlp_forward(hipStream_t stream, const Layout& L, const void* packed, const void* feat, uint32_t N,
                       float* out);
+
+
+// ---- mlp_mfma_train.hip (FP16 MFMA training kernels) ------------------------------
+size_t mfma_packed_bwd_bytes(const NetShape& net);
+void mfma_pack_bwd_weights(hipStream_t stream, const NetShape& net, const float* weights, void* packed);
+// saved hidden activations: half [L][N/16][W][16]
+size_t mfma_hidden_bytes(const Layout& L, uint32_t N);
+// pre-activation gradients: half [L][N/16][W][16] + output layer [N/16][16][16]
+size_t mfma_delta_bytes(const Layout& L, uint32_t N);
+// forward storing hidden activations and the outputs float[N][n_output]
+void mfma_mlp_forward_train(hipStream_t stream, const Layout& L, const void* packed, const void* feat, uint32_t N,
+                            void* hidden, float* out);
+// deltas of every layer from dL/dout float[N][n_output]; dL/denc SoA float [K][N] (nullable)
+void mfma_mlp_backward(hipStream_t stream, const Layout& L, const void* packed_bwd, const void* hidden,
+                       const float* y, const float* dy, uint32_t N, void* delta, float* d_enc);
+size_t mfma_wgrad_workspace_floats(const Layout& L, uint32_t N);
+// network weight gradients (FP32, deterministic) written or added into grads[0 .. n_network_params)
+void mfma_weight_grad(hipStream_t stream, const Layout& L, const void* feat, const void* hidden, const void* delta,
+                      uint32_t N, float* grads, float* workspace, size_t workspace_floats, bool accumulate);
 
 }  // namespace nlod

@@ -64,4 +64,11 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 
 inline uint32_t div_round_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Compute units of the current device.
+int cu_count();
+// Resident blocks of `kernel` per CU at `block` threads and `smem` dynamic LDS
+// bytes (also raises the kernel's dynamic-LDS limit to smem); cached per
+// (device, kernel, block, smem).
+int occupancy_blocks(const void* kernel, int block, size_t smem);
+
 }  // namespace nlod
