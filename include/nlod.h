@@ -184,6 +184,89 @@ int nlod_model_encode_backward(nlod_model* model, void* stream, uint32_t n, cons
 int nlod_model_feature_bytes(const nlod_model* model, uint32_t n, uint64_t* bytes_out);
 int nlod_model_encode_fragments(nlod_model* model, void* stream, uint32_t n, const float* d_in, void* d_features);
 
+/* ---------------------------------------------------------------- trainer */
+
+typedef struct nlod_trainer nlod_trainer;
+
+enum nlod_trainer_flags {
+    NLOD_TRAINER_GRAPH = 1u << 0 /* capture each batch size's step in a hipGraph (after one eager step) */
+};
+
+/* Trainer<float, T, T>(model, create_optimizer(json), create_loss(json), seed)
+ * (tiny-cuda-nn include/tiny-cuda-nn/trainer.h:53-60; src/optimizer.cu:47-80;
+ * src/loss.cu:47-80), as built at rptr/cuda/neural-lod/neural_lod_learning_init.cu:655-668,890-903.
+ * Optimizers: Adam (optimizers/adam.h), ExponentialDecay{nested}, Ema{nested}.
+ * Losses: L2, RelativeL2, BinaryCrossEntropy.  Parameters are initialised as
+ * nlod_model_init_params(seed) and bound to the model (the model must outlive
+ * the trainer; its parameters are the trainer's while it lives).
+ * loss_scale_json (NULL = defaults) replaces the reference's static scale of 128
+ * (trainer.h:171) by dynamic loss scaling: {"initial_scale": 1024, "dynamic": true,
+ * "growth_factor": 2, "backoff_factor": 0.5, "growth_interval": 2000,
+ * "min_scale": 1, "max_scale": 2^32}.  A step whose FP64 gradient check finds a
+ * non-finite value is skipped (no optimizer step) and the scale backs off. */
+int nlod_trainer_create(nlod_model* model, const char* optimizer_json, const char* loss_json,
+                        const char* loss_scale_json, uint32_t seed, uint32_t flags, nlod_trainer** trainer_out);
+int nlod_trainer_destroy(nlod_trainer* trainer);
+
+/* One training step (Trainer::training_step, trainer.h:161-186, with the
+ * optimizer step): d_in float[n][input_width], d_target float[n][output_width].
+ * Asynchronous on `stream`; no host synchronisation inside. */
+int nlod_trainer_step(nlod_trainer* trainer, void* stream, uint32_t n, const float* d_in, const float* d_target);
+
+typedef struct nlod_trainer_status {
+    double loss;          /* loss of the last step (Trainer::loss, trainer.h:197-199), FP64 sum */
+    double grad_norm;     /* L2 norm of the last step's unscaled gradient (FP64) */
+    float loss_scale;     /* scale the next step will use */
+    float learning_rate;  /* learning rate of the last optimizer step */
+    uint32_t step;        /* optimizer steps taken */
+    uint32_t skipped_steps;
+    uint32_t last_step_skipped;
+    uint32_t reserved;
+} nlod_trainer_status;
+/* State after the last step; synchronizes with the trainer's work. */
+int nlod_trainer_get_status(nlod_trainer* trainer, nlod_trainer_status* status_out);
+
+/* Device pointers owned by the trainer: FP32 master parameters
+ * (params_full_precision, trainer.h:217), inference parameters (the EMA weights
+ * when an Ema optimizer is configured, trainer.h:322-325) and gradients. */
+int nlod_trainer_params(nlod_trainer* trainer, float** d_params, float** d_inference_params, float** d_gradients);
+/* Copies one of the trainer's FP32 vectors (0 master parameters, 1 inference
+ * parameters, 2 gradients, 3 first moments, 4 second moments) into the
+ * caller's device buffer float[n_params], ordered after the trainer's work. */
+int nlod_trainer_copy(nlod_trainer* trainer, void* stream, int which, float* d_dst);
+/* set_params_full_precision (trainer.h:231-245): copies d_params into the
+ * master and inference parameters. */
+int nlod_trainer_set_params(nlod_trainer* trainer, void* stream, const float* d_params);
+
+/* serialize / deserialize (trainer.h:267-307) as a binary snapshot: the
+ * inference parameters, plus master parameters, moments, per-parameter step
+ * counts and scaler state when with_optimizer_state.  buf NULL: size query. */
+int nlod_trainer_serialize(nlod_trainer* trainer, int with_optimizer_state, void* buf, size_t buf_len,
+                           size_t* needed);
+int nlod_trainer_deserialize(nlod_trainer* trainer, const void* buf, size_t len);
+
+/* Loss evaluation alone (Loss::evaluate with stride == dims):
+ * values (nullable) and loss_scale * dL/dpred per element, n_total = n_elements.
+ * loss_type: 0 L2 (losses/l2.h), 1 RelativeL2 (losses/relative_l2.h:39-76),
+ * 2 BinaryCrossEntropy (losses/binary_cross_entropy.h:7-43). */
+int nlod_loss_evaluate(void* stream, int loss_type, uint64_t n_elements, float loss_scale, const float* d_pred,
+                       const float* d_target, float* d_values, float* d_gradients);
+
+/* Adam hyperparameters (optimizers/adam.h:214-262). */
+typedef struct nlod_adam_config {
+    float learning_rate, beta1, beta2, epsilon, l2_reg;
+    float relative_decay, absolute_decay, clipping_magnitude, non_matrix_learning_rate_factor;
+    int optimize_matrix_params, optimize_non_matrix_params, adabound;
+} nlod_adam_config;
+
+/* One Adam step on caller buffers (adam_step, optimizers/adam.h:48-119;
+ * optimizer_step = the optimizer's step count after this step, used by
+ * AdaBound).  Gradients are divided by loss_scale; the first n_matrix
+ * parameters are matrix weights.  Synchronizes the stream. */
+int nlod_adam_step(void* stream, const nlod_adam_config* config, uint64_t n, uint64_t n_matrix, float loss_scale,
+                   uint32_t optimizer_step, float* d_weights, float* d_gradients, float* d_first_moments,
+                   float* d_second_moments, uint32_t* d_param_steps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,17 @@ SIGNATURES: dict[str, list] = {
     "nlod_model_encode_backward": [_vp, _vp, _u32, _vp, _vp, _vp, _i32],
     "nlod_model_feature_bytes": [_vp, _u32, C.POINTER(_u64)],
     "nlod_model_encode_fragments": [_vp, _vp, _u32, _vp, _vp],
+    "nlod_trainer_create": [_vp, C.c_char_p, C.c_char_p, C.c_char_p, _u32, _u32, C.POINTER(_vp)],
+    "nlod_trainer_destroy": [_vp],
+    "nlod_trainer_step": [_vp, _vp, _u32, _vp, _vp],
+    "nlod_trainer_get_status": [_vp, _vp],
+    "nlod_trainer_params": [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)],
+    "nlod_trainer_copy": [_vp, _vp, _i32, _vp],
+    "nlod_trainer_set_params": [_vp, _vp, _vp],
+    "nlod_trainer_serialize": [_vp, _i32, _vp, C.c_size_t, C.POINTER(C.c_size_t)],
+    "nlod_trainer_deserialize": [_vp, _vp, C.c_size_t],
+    "nlod_loss_evaluate": [_vp, _i32, _u64, C.c_float, _vp, _vp, _vp, _vp],
+    "nlod_adam_step": [_vp, _vp, _u64, _u64, C.c_float, _u32, _vp, _vp, _vp, _vp, _vp],
 }
 
 
@@ -93,6 +104,28 @@ class ModelInfo(C.Structure):
         ("precision", C.c_uint32),
         ("reserved", C.c_uint32),
     ]
+
+
+class TrainerStatus(C.Structure):
+    _fields_ = [
+        ("loss", C.c_double),
+        ("grad_norm", C.c_double),
+        ("loss_scale", C.c_float),
+        ("learning_rate", C.c_float),
+        ("step", C.c_uint32),
+        ("skipped_steps", C.c_uint32),
+        ("last_step_skipped", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class AdamConfig(C.Structure):
+    _fields_ = [(k, C.c_float) for k in ("learning_rate", "beta1", "beta2", "epsilon", "l2_reg", "relative_decay",
+                                          "absolute_decay", "clipping_magnitude",
+                                          "non_matrix_learning_rate_factor")] + \
+               [(k, C.c_int) for k in ("optimize_matrix_params", "optimize_non_matrix_params", "adabound")]
+
+
 _RESTYPES = {"nlod_last_error": C.c_char_p, "nlod_version": C.c_char_p}
 
 

@@ -27,20 +27,30 @@ void Model::set_params(hipStream_t s, const float* p, const float* p_inf) {
     NLOD_CHECK(p != nullptr, "params pointer is null");
     params = p;
     params_inference = p_inf ? p_inf : p;
-    if (precision == PREC_FP16) {
-        const size_t bytes = mfma_packed_bytes(L.net), bwd_bytes = mfma_packed_bwd_bytes(L.net);
-        packed.ensure(bytes);
-        packed_inference.ensure(bytes);
-        packed_bwd.ensure(bwd_bytes);
-        packed_bwd_inference.ensure(bwd_bytes);
-        mfma_pack_weights(s, L.net, params, packed.p);
-        mfma_pack_weights(s, L.net, params_inference, packed_inference.p);
-        mfma_pack_bwd_weights(s, L.net, params, packed_bwd.p);
-        mfma_pack_bwd_weights(s, L.net, params_inference, packed_bwd_inference.p);
-        table16.ensure(sizeof(uint16_t) * L.n_params);
+    inference_aliased = params_inference == params;
+    if (precision != PREC_FP16) return;
+    packed.ensure(mfma_packed_bytes(L.net));
+    packed_bwd.ensure(mfma_packed_bwd_bytes(L.net));
+    table16.ensure(sizeof(uint16_t) * L.n_params);
+    mfma_to_half(s, params, table16.p, L.n_params);
+    if (!inference_aliased) {
+        packed_inference.ensure(mfma_packed_bytes(L.net));
+        packed_bwd_inference.ensure(mfma_packed_bwd_bytes(L.net));
         table16_inference.ensure(sizeof(uint16_t) * L.n_params);
-        mfma_to_half(s, params, table16.p, L.n_params);
         mfma_to_half(s, params_inference, table16_inference.p, L.n_params);
+    }
+    refresh_packed(s, true, !inference_aliased);
+}
+
+void Model::refresh_packed(hipStream_t s, bool training, bool inference) {
+    if (precision != PREC_FP16) return;
+    if (training) {
+        mfma_pack_weights(s, L.net, params, packed.p);
+        mfma_pack_bwd_weights(s, L.net, params, packed_bwd.p);
+    }
+    if (inference && !inference_aliased) {
+        mfma_pack_weights(s, L.net, params_inference, packed_inference.p);
+        mfma_pack_bwd_weights(s, L.net, params_inference, packed_bwd_inference.p);
     }
 }
 
@@ -103,25 +113,30 @@ void Model::forward_saved(hipStream_t s, uint32_t N, const float* in, bool use_i
 
 std::unique_ptr<ForwardCtx> Model::forward(hipStream_t s, uint32_t N, const float* in, float* out, bool use_inference,
                                            bool prepare_input_gradients) {
-    require_params(use_inference);
     auto ctx = std::make_unique<ForwardCtx>();
-    ctx->N = N;
-    ctx->use_inference_params = use_inference;
-    ctx->prepare_input_gradients = prepare_input_gradients;
-    if (N == 0) return ctx;
-    const size_t out_bytes = sizeof(float) * (size_t)L.net.n_output * N;
-    ctx->out.ensure(out_bytes);
-    ctx->recompute = N > kRecomputeAbove;
-    if (ctx->recompute) {
-        // activations are recomputed chunk by chunk in backward
-        forward_outputs(s, N, in, ctx->out.as<float>(), use_inference);
-    } else {
-        ctx->enc.ensure(enc_bytes(L, precision, N));
-        ctx->hidden.ensure(hidden_bytes(L, precision, N));
-        forward_saved(s, N, in, use_inference, ctx->enc.p, ctx->hidden.p, ctx->out.as<float>());
-    }
-    if (out) NLOD_HIP(hipMemcpyAsync(out, ctx->out.p, out_bytes, hipMemcpyDeviceToDevice, s));
+    forward_into(s, N, in, out, use_inference, prepare_input_gradients, *ctx);
     return ctx;
+}
+
+void Model::forward_into(hipStream_t s, uint32_t N, const float* in, float* out, bool use_inference,
+                         bool prepare_input_gradients, ForwardCtx& ctx) {
+    require_params(use_inference);
+    ctx.N = N;
+    ctx.use_inference_params = use_inference;
+    ctx.prepare_input_gradients = prepare_input_gradients;
+    ctx.recompute = N > kRecomputeAbove;
+    if (N == 0) return;
+    const size_t out_bytes = sizeof(float) * (size_t)L.net.n_output * N;
+    ctx.out.ensure(out_bytes);
+    if (ctx.recompute) {
+        // activations are recomputed chunk by chunk in backward
+        forward_outputs(s, N, in, ctx.out.as<float>(), use_inference);
+    } else {
+        ctx.enc.ensure(enc_bytes(L, precision, N));
+        ctx.hidden.ensure(hidden_bytes(L, precision, N));
+        forward_saved(s, N, in, use_inference, ctx.enc.p, ctx.hidden.p, ctx.out.as<float>());
+    }
+    if (out) NLOD_HIP(hipMemcpyAsync(out, ctx.out.p, out_bytes, hipMemcpyDeviceToDevice, s));
 }
 
 void Model::backward_saved(hipStream_t s, uint32_t N, bool use_inference, const float* in, const float* d_out,
@@ -158,14 +173,14 @@ void Model::backward_saved(hipStream_t s, uint32_t N, bool use_inference, const 
 }
 
 void Model::backward(hipStream_t s, ForwardCtx& ctx, const float* in, const float* d_out, float* d_params, float* d_in,
-                     int grad_mode) {
+                     int grad_mode, bool grid_grads_zero) {
     NLOD_CHECK(grad_mode >= GRAD_IGNORE && grad_mode <= GRAD_ACCUMULATE, "invalid gradient mode");
     NLOD_CHECK(grad_mode == GRAD_IGNORE || d_params, "parameter-gradient pointer is null");
     if (d_in && !encode_backward_input_supported(L))
         fail(NLOD_ERR_UNSUPPORTED, "input gradients through a grid encoding are not supported");
     const uint32_t N = ctx.N;
     const bool param_grads = grad_mode != GRAD_IGNORE;
-    if (grad_mode == GRAD_OVERWRITE && L.n_params > L.n_net_params)
+    if (grad_mode == GRAD_OVERWRITE && L.n_params > L.n_net_params && !grid_grads_zero)
         fill_f32(s, d_params + L.n_net_params, L.n_params - L.n_net_params, 0.0f);
     if (N == 0) {
         if (grad_mode == GRAD_OVERWRITE) fill_f32(s, d_params, L.n_net_params, 0.0f);
@@ -240,6 +255,13 @@ struct nlod_ctx {
     std::unique_ptr<nlod::ForwardCtx> c;
     nlod_model* owner = nullptr;
 };
+
+namespace nlod {
+Model& model_of(nlod_model* m) {
+    NLOD_CHECK(m && m->m, "null model");
+    return *m->m;
+}
+}  // namespace nlod
 
 extern "C" {
 
@@ -338,7 +360,7 @@ int nlod_model_encode_fragments(nlod_model* model, void* stream, uint32_t n, con
         nlod::Model& m = *model->m;
         if (m.precision != nlod::PREC_FP16) nlod::fail(NLOD_ERR_UNSUPPORTED, "FP16 models only");
         m.require_params(true);
-        nlod::mfma_encode(nlod::as_stream(stream), m.L, m.program.p, m.table16_inference.p, d_in, n, d_features);
+        nlod::mfma_encode(nlod::as_stream(stream), m.L, m.program.p, m.table_half(true), d_in, n, d_features);
     });
 }
 

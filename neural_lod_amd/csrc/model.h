@@ -58,13 +58,13 @@ struct Model {
     int precision = PREC_FP32;
     const float* params = nullptr;
     const float* params_inference = nullptr;
-    // FP16 MFMA weights (network part), packed for the fused kernels, and the
-    // device copy of the encoding description they read
-    DevBuf packed, packed_inference, program;
-    // FP16 MFMA transposed weights for the backward products
-    DevBuf packed_bwd, packed_bwd_inference;
-    // FP16 copies of the parameter vectors (the grid tables the encoding pass gathers)
-    DevBuf table16, table16_inference;
+    bool inference_aliased = true;  // params_inference == params: one set of FP16 copies
+    // FP16 form: MLP weights packed for the MFMA kernels (forward and transposed for
+    // backward), the device copy of the encoding description, and FP16 copies of
+    // the whole parameter vector (the grid tables the encoding pass gathers)
+    DevBuf packed, packed_bwd, table16;
+    DevBuf packed_inference, packed_bwd_inference, table16_inference;
+    DevBuf program;
     DevBuf ws_feat;
     // scratch
     DevBuf ws_enc, ws_delta, ws_delta_out, ws_denc, ws_wgrad;
@@ -74,18 +74,26 @@ struct Model {
     Model(uint32_t n_in, uint32_t n_out, const Json& enc, const Json& net, int prec);
 
     void set_params(hipStream_t s, const float* p, const float* p_inf);
+    // FP16 form: repack the MLP weights after the FP32 parameters changed on the
+    // device (the FP16 table copies are written by whoever changed them)
+    void refresh_packed(hipStream_t s, bool training, bool inference);
     void require_params(bool inference) const;
     const float* weights(bool inference) const { return inference ? params_inference : params; }
+    bool use_inference_copy(bool inference) const { return inference && !inference_aliased; }
+    const void* packed_fwd(bool inf) const { return use_inference_copy(inf) ? packed_inference.p : packed.p; }
+    const void* packed_back(bool inf) const { return use_inference_copy(inf) ? packed_bwd_inference.p : packed_bwd.p; }
+    void* table_half(bool inf) const { return use_inference_copy(inf) ? table16_inference.p : table16.p; }
 
     void encode(hipStream_t s, uint32_t N, const float* in, float* enc_soa, bool inference);
     void inference(hipStream_t s, uint32_t N, const float* in, float* out);
     std::unique_ptr<ForwardCtx> forward(hipStream_t s, uint32_t N, const float* in, float* out, bool use_inference,
                                         bool prepare_input_gradients);
+    // forward into an existing context (its buffers only grow)
+    void forward_into(hipStream_t s, uint32_t N, const float* in, float* out, bool use_inference,
+                      bool prepare_input_gradients, ForwardCtx& ctx);
+    // grid_grads_zero: the grid part of d_params is already zero (skips its clearing under OVERWRITE)
     void backward(hipStream_t s, ForwardCtx& ctx, const float* in, const float* d_out, float* d_params,
-                  float* d_in, int grad_mode);
-    const void* packed_fwd(bool inference) const { return inference ? packed_inference.p : packed.p; }
-    const void* packed_back(bool inference) const { return inference ? packed_bwd_inference.p : packed_bwd.p; }
-    const void* table_half(bool inference) const { return inference ? table16_inference.p : table16.p; }
+                  float* d_in, int grad_mode, bool grid_grads_zero = false);
 
   private:
     // outputs only (no saved activations), chunked

@@ -87,3 +87,22 @@ def throughput_queries(n: int, n_lods: int = 5, seed: int = 2) -> np.ndarray:
     pos = rng.uniform(0.0, 1.0, size=(n, 3))
     lod = rng.integers(0, n_lods, size=(n, 1)).astype(np.float64)
     return np.ascontiguousarray(np.concatenate([pos, dirs(), dirs(), lod], axis=1).astype(np.float32))
+
+
+def sphere_visibility_labels(queries: np.ndarray, centre=(0.5, 0.5, 0.5), radius: float = 0.25,
+                             max_dist: float = 0.35) -> np.ndarray:
+    """Analytic visibility targets for visibility queries [pos(3), (dir+1)/2 (3)]:
+    1.0 where the ray from pos along dir hits the sphere within max_dist
+    (a stand-in for "hit before leaving the voxel"), else 0.0.  float32 [n][1]."""
+    q = np.asarray(queries, np.float64)
+    o = q[:, :3] - np.asarray(centre, np.float64)
+    d = q[:, 3:6] * 2.0 - 1.0
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    b = (o * d).sum(axis=1)
+    c = (o * o).sum(axis=1) - radius * radius
+    disc = b * b - c
+    sq = np.sqrt(np.maximum(disc, 0.0))
+    t0, t1 = -b - sq, -b + sq
+    t = np.where(t0 > 0.0, t0, t1)  # nearest hit in front of the origin (inside: the exit)
+    hit = (disc >= 0.0) & (t > 0.0) & (t <= max_dist)
+    return hit.astype(np.float32)[:, None]

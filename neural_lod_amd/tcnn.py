@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import ModelInfo
+from ._lib import AdamConfig, ModelInfo, TrainerStatus
 
 FP32 = 0
 FP16 = 1
@@ -181,6 +181,113 @@ class NetworkWithInputEncoding:
         _lib.call("nlod_model_backward", self._h, _stream(x.device), ctx._h, _ptr(x), _ptr(d_out), _ptr(d_params),
                   _ptr(d_in), mode)
         return d_params, d_in
+
+
+LOSS_L2, LOSS_RELATIVE_L2, LOSS_BCE = 0, 1, 2
+TRAINER_GRAPH = 1
+
+
+class Trainer:
+    """tcnn ``Trainer`` (include/tiny-cuda-nn/trainer.h): FP32 master
+    parameters, Adam (+ ExponentialDecay / Ema wrappers from the optimizer
+    JSON), a loss from the loss JSON, and dynamic loss scaling.  The model's
+    parameters are bound to the trainer's buffers while the trainer lives."""
+
+    def __init__(self, model: NetworkWithInputEncoding, optimizer_config, loss_config, seed: int = 1337,
+                 loss_scale_config=None, graphs: bool = False):
+        h = C.c_void_p()
+        _lib.call("nlod_trainer_create", model._h, _as_json(optimizer_config), _as_json(loss_config),
+                  _as_json(loss_scale_config) if loss_scale_config is not None else None, seed,
+                  TRAINER_GRAPH if graphs else 0, C.byref(h))
+        self._h = h
+        self.model = model
+        model._params = None
+        model._inference_params = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None and self._h.value:
+                _lib.lib().nlod_trainer_destroy(self._h)
+        except Exception:  # interpreter shutdown
+            pass
+        self._h = None
+
+    def training_step(self, x: torch.Tensor, target: torch.Tensor) -> None:
+        """One forward + loss + backward + optimizer step (asynchronous)."""
+        _check_f32(x, "input", (x.shape[0], self.model.n_input_dims))
+        _check_f32(target, "target", (x.shape[0], self.model.n_output_dims))
+        _lib.call("nlod_trainer_step", self._h, _stream(x.device), x.shape[0], _ptr(x), _ptr(target))
+
+    def status(self) -> dict:
+        st = TrainerStatus()
+        _lib.call("nlod_trainer_get_status", self._h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in TrainerStatus._fields_ if k != "reserved"}
+
+    def loss(self) -> float:
+        """Loss of the last step (synchronizes)."""
+        return self.status()["loss"]
+
+    def _copy(self, which: int) -> torch.Tensor:
+        out = torch.empty(self.model.n_params, dtype=torch.float32, device="cuda")
+        _lib.call("nlod_trainer_copy", self._h, _stream(out.device), which, _ptr(out))
+        return out
+
+    def params(self) -> torch.Tensor:
+        return self._copy(0)
+
+    def inference_params(self) -> torch.Tensor:
+        return self._copy(1)
+
+    def gradients(self) -> torch.Tensor:
+        return self._copy(2)
+
+    def moments(self) -> tuple[torch.Tensor, torch.Tensor]:
+        return self._copy(3), self._copy(4)
+
+    def set_params(self, params: torch.Tensor) -> None:
+        _check_f32(params, "params", (self.model.n_params,))
+        _lib.call("nlod_trainer_set_params", self._h, _stream(params.device), _ptr(params))
+
+    def serialize(self, with_optimizer_state: bool = False) -> bytes:
+        need = C.c_size_t()
+        _lib.call("nlod_trainer_serialize", self._h, int(with_optimizer_state), None, 0, C.byref(need))
+        buf = C.create_string_buffer(need.value)
+        _lib.call("nlod_trainer_serialize", self._h, int(with_optimizer_state), buf, need.value, C.byref(need))
+        return buf.raw
+
+    def deserialize(self, blob: bytes) -> None:
+        _lib.call("nlod_trainer_deserialize", self._h, C.c_char_p(blob), len(blob))
+
+
+def loss_evaluate(loss_type: int, pred: torch.Tensor, target: torch.Tensor, loss_scale: float = 1.0):
+    """Per-element loss values and scaled gradients (tcnn Loss::evaluate)."""
+    _check_f32(pred, "pred")
+    _check_f32(target, "target", tuple(pred.shape))
+    values = torch.empty_like(pred)
+    grads = torch.empty_like(pred)
+    _lib.call("nlod_loss_evaluate", _stream(pred.device), loss_type, pred.numel(), loss_scale, _ptr(pred),
+              _ptr(target), _ptr(values), _ptr(grads))
+    return values, grads
+
+
+def adam_step(config: dict, weights, gradients, m, v, steps, n_matrix: int, loss_scale: float,
+              optimizer_step: int) -> None:
+    """One Adam step in place on device tensors (tcnn adam_step kernel)."""
+    cfg = AdamConfig(learning_rate=config.get("learning_rate", 1e-3), beta1=config.get("beta1", 0.9),
+                     beta2=config.get("beta2", 0.999), epsilon=config.get("epsilon", 1e-8),
+                     l2_reg=config.get("l2_reg", 1e-8), relative_decay=config.get("relative_decay", 0.0),
+                     absolute_decay=config.get("absolute_decay", 0.0),
+                     clipping_magnitude=config.get("clipping_magnitude", 0.0),
+                     non_matrix_learning_rate_factor=config.get("non_matrix_learning_rate_factor", 1.0),
+                     optimize_matrix_params=int(config.get("optimize_matrix_params", True)),
+                     optimize_non_matrix_params=int(config.get("optimize_non_matrix_params", True)),
+                     adabound=int(config.get("adabound", False)))
+    for t, name in ((weights, "weights"), (gradients, "gradients"), (m, "m"), (v, "v")):
+        _check_f32(t, name, (weights.numel(),))
+    if not (steps.is_cuda and steps.dtype == torch.int32 and steps.numel() == weights.numel()):
+        raise TypeError("steps must be an int32 CUDA tensor of the weights' size")
+    _lib.call("nlod_adam_step", _stream(weights.device), C.byref(cfg), weights.numel(), n_matrix, loss_scale,
+              optimizer_step, _ptr(weights), _ptr(gradients), _ptr(m), _ptr(v), _ptr(steps))
 
 
 class _ModuleFunction(torch.autograd.Function):

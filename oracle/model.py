@@ -244,8 +244,9 @@ def _lib():
         L.nlo_grid_corners.argtypes = [sp, fp, C.POINTER(C.c_int64)]
         L.nlo_infer_f32.argtypes = [sp, C.c_int, np_, fp, fp, C.c_int, C.c_int64, fp]
         L.nlo_loss.argtypes = [C.c_int, dp, fp, C.c_int64, C.c_int, C.c_int, C.c_double, dp, dp]
-        L.nlo_adam_step.argtypes = [C.c_int64, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
-                                    C.c_float, fp, fp, fp, fp, C.POINTER(C.c_uint32)]
+        L.nlo_adam_step.argtypes = [C.c_int64, C.c_int64, C.c_float, C.POINTER(AdamCfg), C.c_uint32, fp, fp, fp, fp,
+                                    C.POINTER(C.c_uint32)]
+        L.nlo_ema_step.argtypes = [C.c_int64, C.c_float, C.c_uint32, fp, fp]
         L.nlo_omp_threads.restype = C.c_int
         _L = L
     return _L
@@ -331,10 +332,103 @@ def loss(kind: str, pred: np.ndarray, target: np.ndarray, n_output: int, loss_sc
     return values, grads
 
 
-def adam_step(w, grad, m, v, steps, n_matrix, loss_scale, lr, beta1, beta2, eps, l2_reg):
-    """In-place float32 Adam step (adam.h:48-119)."""
-    _lib().nlo_adam_step(w.size, n_matrix, loss_scale, lr, beta1, beta2, eps, l2_reg, _fp(w), _fp(grad), _fp(m),
-                         _fp(v), steps.ctypes.data_as(C.POINTER(C.c_uint32)))
+class AdamCfg(C.Structure):
+    _fields_ = [(k, C.c_float) for k in ("lr", "beta1", "beta2", "eps", "l2_reg", "relative_decay", "absolute_decay",
+                                          "clipping", "non_matrix_lr_factor")] + \
+               [(k, C.c_int) for k in ("optimize_matrix", "optimize_non_matrix", "adabound")]
+
+
+def adam_config(cfg: dict) -> AdamCfg:
+    """tcnn Adam JSON keys (adam.h:214-262) with tcnn's defaults."""
+    return AdamCfg(lr=cfg.get("learning_rate", 1e-3), beta1=cfg.get("beta1", 0.9), beta2=cfg.get("beta2", 0.999),
+                   eps=cfg.get("epsilon", 1e-8), l2_reg=cfg.get("l2_reg", 1e-8),
+                   relative_decay=cfg.get("relative_decay", 0.0), absolute_decay=cfg.get("absolute_decay", 0.0),
+                   clipping=cfg.get("clipping_magnitude", 0.0),
+                   non_matrix_lr_factor=cfg.get("non_matrix_learning_rate_factor", 1.0),
+                   optimize_matrix=int(cfg.get("optimize_matrix_params", True)),
+                   optimize_non_matrix=int(cfg.get("optimize_non_matrix_params", True)),
+                   adabound=int(cfg.get("adabound", False)))
+
+
+def adam_step(w, grad, m, v, steps, n_matrix, loss_scale, lr, beta1, beta2, eps, l2_reg, opt_step=1, **extra):
+    """In-place float32 Adam step (adam.h:48-119); extra = other tcnn Adam keys."""
+    cfg = dict(learning_rate=lr, beta1=beta1, beta2=beta2, epsilon=eps, l2_reg=l2_reg, **extra)
+    a = adam_config(cfg)
+    _lib().nlo_adam_step(w.size, n_matrix, loss_scale, C.byref(a), opt_step, _fp(w), _fp(grad), _fp(m), _fp(v),
+                         steps.ctypes.data_as(C.POINTER(C.c_uint32)))
+
+
+def ema_step(w, ema, decay, t):
+    """In-place EMA (ema.h, full precision); t = optimizer steps taken so far."""
+    _lib().nlo_ema_step(w.size, decay, t, _fp(w), _fp(ema))
+
+
+def exp_decay_factors(n_steps: int, base: float, start: int, interval: int, end: int) -> np.ndarray:
+    """Learning-rate factor of each step 0..n-1 (ExponentialDecayOptimizer::step,
+    exponential_decay.h:60-71): the float factor is multiplied by base before
+    the step whose index s satisfies s >= start, (s - start) % interval == 0, s <= end."""
+    f = np.float32(1.0)
+    out = np.empty(n_steps, np.float32)
+    for s in range(n_steps):
+        if s >= start and (s - start) % interval == 0 and s <= end:
+            f = np.float32(f * np.float32(base))
+        out[s] = f
+    return out
+
+
+def unwrap_optimizer(cfg: dict) -> dict:
+    """Flattens Ema{ExponentialDecay{Adam}} style JSON into one dict of settings."""
+    out = {"adam": {}, "exp_decay": None, "ema": None}
+    node = cfg
+    while True:
+        t = node.get("otype", "Adam").lower()
+        if t == "adam":
+            out["adam"] = node
+            return out
+        if t == "exponentialdecay":
+            out["exp_decay"] = dict(base=node.get("decay_base", 0.1), start=node.get("decay_start", 10000),
+                                    interval=node.get("decay_interval", 10000), end=node.get("decay_end", 10000000))
+        elif t == "ema":
+            out["ema"] = node.get("decay", 0.99)
+        else:
+            raise ValueError(f"optimizer {t} not restated")
+        node = node.get("nested", {})
+
+
+def train(layout: Layout, params: np.ndarray, batches, optimizer: dict, loss_kind: str, loss_scale: float = 1.0):
+    """The training loop of tcnn's Trainer restated with the FP64 forward /
+    backward and the float32 Adam above: for each (inputs, targets) batch,
+    forward, loss (scaled by loss_scale), backward, optimizer step.  Returns
+    (losses per step, final master params, final inference params)."""
+    o = unwrap_optimizer(optimizer)
+    a = o["adam"]
+    lr0 = a.get("learning_rate", 1e-3)
+    w = np.array(params, np.float32)
+    n = w.size
+    m = np.zeros(n, np.float32)
+    v = np.zeros(n, np.float32)
+    steps = np.zeros(n, np.uint32)
+    ema = np.zeros(n, np.float32) if o["ema"] is not None else None
+    decay = o["exp_decay"]
+    factors = exp_decay_factors(len(batches), decay["base"], decay["start"], decay["interval"], decay["end"]) \
+        if decay else np.ones(len(batches), np.float32)
+    losses = []
+    for s, (x, t) in enumerate(batches):
+        out = forward(layout, w, x)
+        vals, g_out = loss(loss_kind, out[:, :layout.net.n_output], t, layout.net.n_output, loss_scale)
+        losses.append(vals.sum())
+        d = np.zeros((x.shape[0], layout.net.n_output_padded))
+        d[:, :layout.net.n_output] = g_out
+        g, _ = backward(layout, w, x, d)
+        lr = np.float32(np.float32(lr0) * factors[s])
+        extra = {k: a[k] for k in ("relative_decay", "absolute_decay", "clipping_magnitude",
+                                   "non_matrix_learning_rate_factor", "optimize_matrix_params",
+                                   "optimize_non_matrix_params", "adabound") if k in a}
+        adam_step(w, g.astype(np.float32), m, v, steps, layout.n_net_params, loss_scale, lr, a.get("beta1", 0.9),
+                  a.get("beta2", 0.999), a.get("epsilon", 1e-8), a.get("l2_reg", 1e-8), opt_step=s + 1, **extra)
+        if ema is not None:
+            ema_step(w, ema, o["ema"], s + 1)
+    return np.array(losses), w, (ema if ema is not None else w.copy())
 
 
 def grid_corners(seg: Seg, x) -> np.ndarray:

@@ -27,6 +27,7 @@
  *                            losses/relative_l2.h:39-76
  *   Adam                  <- optimizers/adam.h:48-119 (float32, op by op)
  */
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -489,8 +490,10 @@ void nlo_loss(int type, const double* pred, const float* target, int64_t N, int 
             double t = target[i * dims + d];
             double p = pred[k];
             if (type == LOSS_BCE) {
-                const double eps = 1e-5;
-                p = p < eps ? eps : (p > 1.0 - eps ? 1.0 - eps : p);
+                /* binary_cross_entropy.h:33-34 clamps the float prediction to [1e-5f, 1 - 1e-5f] */
+                const float eps = 1e-5f;
+                const float pf = fminf(fmaxf((float)p, eps), 1.0f - eps);
+                p = (double)pf;
                 values[k] = -(t * log(p) + (1.0 - t) * log(1.0 - p)) / n_total;
                 grads[k] = loss_scale * (-(t - p) / (p - p * p)) / n_total;
             } else if (type == LOSS_REL_L2) {
@@ -509,23 +512,52 @@ void nlo_loss(int type, const double* pred, const float* target, int64_t N, int 
 
 /* -------------------------------------------------------------- Adam */
 
-/* adam.h:48-119 in float32, one parameter at a time (no weight decay,
- * clipping or AdaBound terms: the path's configs leave them at 0/false). */
-void nlo_adam_step(int64_t n, int64_t n_matrix, float loss_scale, float lr, float beta1, float beta2, float eps,
-                   float l2_reg, float* w, const float* grad, float* m, float* v, uint32_t* steps) {
+typedef struct nlo_adam {
+    float lr, beta1, beta2, eps, l2_reg;
+    float relative_decay, absolute_decay, clipping, non_matrix_lr_factor;
+    int optimize_matrix, optimize_non_matrix, adabound;
+} nlo_adam;
+
+/* adam_step (optimizers/adam.h:48-119) in float32, one parameter at a time;
+ * AdamOptimizer::step (:150-178) supplies the AdaBound bounds from the
+ * optimizer's step count after its increment (opt_step). */
+void nlo_adam_step(int64_t n, int64_t n_matrix, float loss_scale, const nlo_adam* a, uint32_t opt_step, float* w,
+                   const float* grad, float* m, float* v, uint32_t* steps) {
+    float lo = 0.0f, hi = FLT_MAX;
+    if (a->adabound) {
+        lo = 0.1f - 0.1f / ((1.0f - a->beta2) * (float)opt_step + 1.0f);
+        hi = 0.1f + 0.1f / ((1.0f - a->beta2) * (float)opt_step);
+    }
     for (int64_t i = 0; i < n; ++i) {
         float g = grad[i] / loss_scale;
-        if (i >= n_matrix && g == 0.0f) continue;
+        if (i >= n_matrix) {
+            if (!a->optimize_non_matrix || g == 0.0f) continue;
+        } else if (!a->optimize_matrix) {
+            continue;
+        }
         const float wf = w[i];
-        if (i < n_matrix) g += l2_reg * wf;
+        if (i < n_matrix) g += a->l2_reg * wf;
         const float g2 = g * g;
-        m[i] = beta1 * m[i] + (1.0f - beta1) * g;
-        v[i] = beta2 * v[i] + (1.0f - beta2) * g2;
+        m[i] = a->beta1 * m[i] + (1.0f - a->beta1) * g;
+        v[i] = a->beta2 * v[i] + (1.0f - a->beta2) * g2;
+        float lr = a->lr;
+        if (i >= n_matrix) lr *= a->non_matrix_lr_factor;
         const uint32_t t = ++steps[i];
-        float lr_t = lr * sqrtf(1.0f - powf(beta2, (float)t)) / (1.0f - powf(beta1, (float)t));
-        float eff = lr_t / (sqrtf(v[i]) + eps);
-        w[i] = wf - eff * m[i];
+        lr *= sqrtf(1.0f - powf(a->beta2, (float)t)) / (1.0f - powf(a->beta1, (float)t));
+        const float eff = fminf(fmaxf(lr / (sqrtf(v[i]) + a->eps), lo), hi);
+        const float decayed = (1.0f - a->relative_decay * lr) * wf - copysignf(a->absolute_decay * lr, wf);
+        float nw = decayed - eff * m[i];
+        if (a->clipping != 0.0f) nw = fminf(fmaxf(nw, -a->clipping), a->clipping);
+        w[i] = nw;
     }
+}
+
+/* EmaOptimizer::step (optimizers/ema.h:45-62 and :95-130), full precision:
+ * t = the nested optimizer's step count after its step. */
+void nlo_ema_step(int64_t n, float decay, uint32_t t, const float* w, float* ema) {
+    const float debias_old = 1.0f - (float)pow((double)decay, (double)t - 1.0);
+    const float debias_new = 1.0f / (1.0f - (float)pow((double)decay, (double)t));
+    for (int64_t i = 0; i < n; ++i) ema[i] = (ema[i] * decay * debias_old + w[i] * (1.0f - decay)) * debias_new;
 }
 
 int nlo_omp_threads(void) {

@@ -218,7 +218,9 @@ def test_losses_closed_form():
     n = 3.0
     assert np.isclose(v[0, 0], -np.log(0.7) / n)
     assert np.isclose(v[1, 0], -np.log(0.8) / n)
-    assert np.isclose(v[2, 0], -np.log(1 - 1e-5) / n)  # prediction clamped to 1-1e-5
+    # the prediction is clamped in float32 to 1 - 1e-5 (binary_cross_entropy.h:33-34)
+    clamped = float(np.float32(1.0) - np.float32(1e-5))
+    assert np.isclose(v[2, 0], -np.log(clamped) / n, rtol=1e-12, atol=0)
     assert np.isclose(g[0, 0], 128.0 * (0.7 - 1.0) / (0.7 * 0.3) / n)
     assert v[0, 1] == 0 and g[0, 1] == 0  # padded output column
     # relative L2: (p-t)^2 / (p^2 + 0.01), gradient treats the denominator as constant
