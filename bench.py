@@ -26,6 +26,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 N_QUERIES = 1 << 20
+TRAIN_BATCH = 1 << 18
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # algorithmic bytes per query of the FP16 encoding pass (the dominant kernel):
 # 16 levels x 8 corners x 2 features x 2 B gathered (FP16 table copy),
@@ -46,6 +47,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-train", action="store_true", help="skip the training-step measurement")
+    ap.add_argument("--train-steps", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); run with --gpus 1", file=sys.stderr)
@@ -119,6 +122,42 @@ def main():
     err = {k: float(np.abs(outs[k][:n_chk].cpu().numpy() - ref).max()) for k in outs}
     log(f"max abs error vs FP64 oracle ({n_chk} queries): {err}")
 
+    # training step (forward + BCE + backward + FP64 gradient check + Adam), batch 2^18
+    train = None
+    if not args.no_train:
+        from neural_lod_amd.tcnn import Trainer
+        adam = {"otype": "Adam", "learning_rate": 1e-2, "beta1": 0.9, "beta2": 0.99, "epsilon": 1e-15,
+                "l2_reg": 1e-8}
+        nb = TRAIN_BATCH
+        data = []
+        for k in range(4):  # batches resident in HBM, cycled
+            xq = synthetic.visibility_queries(nb, 100 + k)
+            data.append((torch.from_numpy(xq).to(dev), torch.from_numpy(synthetic.sphere_visibility_labels(xq)).to(dev)))
+        train = {}
+        for name, prec in (("fp32", FP32), ("fp16", FP16)):
+            m = NetworkWithInputEncoding(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"], prec)
+            tr = Trainer(m, adam, {"otype": "BinaryCrossEntropy"}, seed=1337, graphs=True)
+            for k in range(max(args.warmup, 2)):
+                tr.training_step(*data[k % 4])
+            torch.cuda.synchronize()
+            stream = torch.cuda.current_stream()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for k in range(args.train_steps):
+                tr.training_step(*data[k % 4])
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / args.train_steps
+            st = tr.status()
+            train[name] = {"ms_per_step": ms, "steps_per_s": 1e3 / ms, "final_loss": st["loss"],
+                           "loss_scale": st["loss_scale"], "skipped_steps": st["skipped_steps"]}
+            log(f"train {name}: {ms:.4f} ms/step, loss {st['loss']:.4f}, scale {st['loss_scale']}")
+            del tr, m
+        train["speedup_fp16_vs_fp32"] = train["fp32"]["ms_per_step"] / train["fp16"]["ms_per_step"]
+        train["batch"] = nb
+        train["labels"] = "analytic sphere occlusion of random rays (synthetic)"
+
     # CPU baseline: the oracle's FP32 C loop on a bounded sample
     cpu = None
     if not args.no_cpu:
@@ -167,6 +206,7 @@ def main():
                      "mlp_kernel_ms": fp16["ms"] - enc_ms, "mlp_flop_per_query": FLOP_PER_QUERY,
                      "mlp_tflops": FLOP_PER_QUERY * N_QUERIES / ((fp16["ms"] - enc_ms) * 1e-3) / 1e12},
         "cpu_baseline": cpu,
+        "train": train,
     }
     print(json.dumps(line), flush=True)
 
