@@ -16,6 +16,7 @@ namespace nlod {
 Model::Model(uint32_t n_in, uint32_t n_out, const Json& enc, const Json& net, int prec) : precision(prec) {
     if (prec != PREC_FP32 && prec != PREC_FP16) fail(NLOD_ERR_INVALID_ARGUMENT, "precision must be FP32 (0) or FP16 (1)");
     L = build_layout(n_in, n_out, enc, net);
+    if (L.n_params > L.n_net_params) grid_bwd = std::make_unique<GridBackward>(L);
     if (precision == PREC_FP16) {
         if (!mfma_supported(L)) fail(NLOD_ERR_UNSUPPORTED, "this layout is not supported by the FP16 MFMA kernels");
         program.ensure(mfma_program_bytes());
@@ -139,9 +140,19 @@ void Model::forward_into(hipStream_t s, uint32_t N, const float* in, float* out,
     if (out) NLOD_HIP(hipMemcpyAsync(out, ctx.out.p, out_bytes, hipMemcpyDeviceToDevice, s));
 }
 
+void Model::grid_backward(hipStream_t s, const float* in, uint32_t N, const float* d_enc, float* d_params,
+                          bool overwrite) {
+    if (grid_overwrite_covers_tables()) {
+        grid_bwd->run(s, in, N, d_enc, d_params, overwrite);
+        return;
+    }
+    if (overwrite) fill_f32(s, d_params + L.n_net_params, L.n_params - L.n_net_params, 0.0f);
+    encode_backward_params(s, L, in, N, d_enc, d_params);
+}
+
 void Model::backward_saved(hipStream_t s, uint32_t N, bool use_inference, const float* in, const float* d_out,
                            const void* enc, const void* hidden, const float* out, float* d_params, float* d_in,
-                           bool param_grads, bool net_accumulate) {
+                           bool param_grads, bool net_accumulate, bool grid_overwrite) {
     const NetShape& n = L.net;
     const bool grid_grads = param_grads && L.n_params > L.n_net_params;
     const bool need_denc = grid_grads || d_in;
@@ -168,7 +179,7 @@ void Model::backward_saved(hipStream_t s, uint32_t N, bool use_inference, const 
                                 ws_delta_out.as<float>(), N, d_params, ws_wgrad.as<float>(), wsf, net_accumulate);
         }
     }
-    if (grid_grads) encode_backward_params(s, L, in, N, d_enc, d_params);
+    if (grid_grads) grid_backward(s, in, N, d_enc, d_params, grid_overwrite);
     if (d_in) encode_backward_input(s, L, in, N, d_enc, d_in);
 }
 
@@ -180,21 +191,23 @@ void Model::backward(hipStream_t s, ForwardCtx& ctx, const float* in, const floa
         fail(NLOD_ERR_UNSUPPORTED, "input gradients through a grid encoding are not supported");
     const uint32_t N = ctx.N;
     const bool param_grads = grad_mode != GRAD_IGNORE;
-    if (grad_mode == GRAD_OVERWRITE && L.n_params > L.n_net_params && !grid_grads_zero)
-        fill_f32(s, d_params + L.n_net_params, L.n_params - L.n_net_params, 0.0f);
+    const bool has_grid = L.n_params > L.n_net_params;
+    // OVERWRITE: the first batch (chunk) stores the grid gradients, later ones add
+    const bool grid_overwrite = grad_mode == GRAD_OVERWRITE && !(grid_grads_zero && !grid_overwrite_covers_tables());
     if (N == 0) {
-        if (grad_mode == GRAD_OVERWRITE) fill_f32(s, d_params, L.n_net_params, 0.0f);
+        if (grad_mode == GRAD_OVERWRITE) fill_f32(s, d_params, grid_grads_zero ? L.n_net_params : L.n_params, 0.0f);
         return;
     }
     NLOD_CHECK(in && d_out, "input / output-gradient pointer is null");
     const bool use_inf = ctx.use_inference_params;
+    (void)has_grid;
     if (!ctx.recompute) {
         backward_saved(s, N, use_inf, in, d_out, ctx.enc.p, ctx.hidden.p, ctx.out.as<float>(), d_params, d_in,
-                       param_grads, grad_mode == GRAD_ACCUMULATE);
+                       param_grads, grad_mode == GRAD_ACCUMULATE, grid_overwrite);
         return;
     }
-    // recompute the activations of each chunk, then its backward; network
-    // gradients of later chunks are added to the first chunk's
+    // recompute the activations of each chunk, then its backward; gradients of
+    // later chunks are added to the first chunk's
     const uint32_t chunk = kRecomputeAbove;
     ws_feat.ensure(enc_bytes(L, precision, chunk));
     ws_hidden.ensure(hidden_bytes(L, precision, chunk));
@@ -205,7 +218,7 @@ void Model::backward(hipStream_t s, ForwardCtx& ctx, const float* in, const floa
         forward_saved(s, n, in_c, use_inf, ws_feat.p, ws_hidden.p, ws_out.as<float>());
         backward_saved(s, n, use_inf, in_c, d_out + (size_t)q0 * L.net.n_output, ws_feat.p, ws_hidden.p,
                        ws_out.as<float>(), d_params, d_in ? d_in + (size_t)q0 * L.n_input_dims : nullptr, param_grads,
-                       grad_mode == GRAD_ACCUMULATE || q0 > 0);
+                       grad_mode == GRAD_ACCUMULATE || q0 > 0, grid_overwrite && q0 == 0);
     }
 }
 
@@ -341,8 +354,11 @@ int nlod_model_encode_backward(nlod_model* model, void* stream, uint32_t n, cons
         nlod::Model& m = *model->m;
         NLOD_CHECK(grad_mode == nlod::GRAD_OVERWRITE || grad_mode == nlod::GRAD_ACCUMULATE, "invalid gradient mode");
         hipStream_t s = nlod::as_stream(stream);
-        if (grad_mode == nlod::GRAD_OVERWRITE) nlod::fill_f32(s, d_dL_dparams, m.L.n_params, 0.0f);
-        if (n) nlod::encode_backward_params(s, m.L, d_in, n, d_dL_dencoded, d_dL_dparams);
+        const bool overwrite = grad_mode == nlod::GRAD_OVERWRITE;
+        if (overwrite) nlod::fill_f32(s, d_dL_dparams, m.L.n_net_params, 0.0f);
+        if (m.L.n_params == m.L.n_net_params) return;
+        if (n) m.grid_backward(s, d_in, n, d_dL_dencoded, d_dL_dparams, overwrite);
+        else if (overwrite) nlod::fill_f32(s, d_dL_dparams + m.L.n_net_params, m.L.n_params - m.L.n_net_params, 0.0f);
     });
 }
 

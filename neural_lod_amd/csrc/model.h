@@ -7,35 +7,12 @@
 #include <memory>
 #include <vector>
 
+#include "grid_backward.h"
 #include "kernels.h"
 #include "layout.h"
 #include "nlod_common.h"
 
 namespace nlod {
-
-// Grow-only device allocation (never reallocated while a stream may use it
-// within one call; growth happens before launches).
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    void ensure(size_t b) {
-        if (b <= bytes) return;
-        release();
-        NLOD_HIP(hipMalloc(&p, b));
-        bytes = b;
-    }
-    template <typename T>
-    T* as() const { return static_cast<T*>(p); }
-};
 
 enum Precision : int { PREC_FP32 = 0, PREC_FP16 = 1 };
 enum GradMode : int { GRAD_IGNORE = 0, GRAD_OVERWRITE = 1, GRAD_ACCUMULATE = 2 };
@@ -70,6 +47,8 @@ struct Model {
     DevBuf ws_enc, ws_delta, ws_delta_out, ws_denc, ws_wgrad;
     // recomputed activations of one chunk (batches above kRecomputeAbove)
     DevBuf ws_hidden, ws_out;
+    // hash-grid parameter gradients (binned, no global atomics)
+    std::unique_ptr<GridBackward> grid_bwd;
 
     Model(uint32_t n_in, uint32_t n_out, const Json& enc, const Json& net, int prec);
 
@@ -105,10 +84,16 @@ struct Model {
     // network gradients to d_params instead of overwriting them
     void backward_saved(hipStream_t s, uint32_t N, bool use_inference, const float* in, const float* d_out,
                         const void* enc, const void* hidden, const float* out, float* d_params, float* d_in,
-                        bool param_grads, bool net_accumulate);
+                        bool param_grads, bool net_accumulate, bool grid_overwrite);
 
   public:
     void init_params_host(uint32_t seed, float* out) const;
+    // grid gradients from dL/d(enc) SoA [K][N]: overwrite stores every table entry
+    // (untouched ones get 0), otherwise they are added
+    void grid_backward(hipStream_t s, const float* in, uint32_t N, const float* d_enc, float* d_params,
+                       bool overwrite);
+    // true when grid_backward(overwrite) writes every table entry itself
+    bool grid_overwrite_covers_tables() const { return grid_bwd && grid_bwd->supported(); }
 };
 
 // Batches above this size recompute their activations chunk by chunk in the

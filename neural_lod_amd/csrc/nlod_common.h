@@ -64,6 +64,30 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 
 inline uint32_t div_round_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Grow-only device allocation (never reallocated while a stream may use it
+// within one call; growth happens before launches).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(size_t b) {
+        if (b <= bytes) return;
+        release();
+        NLOD_HIP(hipMalloc(&p, b));
+        bytes = b;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
 // Compute units of the current device.
 int cu_count();
 // Resident blocks of `kernel` per CU at `block` threads and `smem` dynamic LDS

@@ -190,12 +190,15 @@ struct Trainer {
         const uint64_t n_out = (uint64_t)N * model.L.net.n_output;
         loss_forward(s, loss_type, n_out, model.L.net.n_output, state.p(), ctx.out.as<float>(), target,
                      d_out.as<float>(), loss_partial.p());
-        // the gradient buffer is all zero here (cleared by the previous optimizer step)
+        // the grid part of the gradient buffer is all zero here unless the grid
+        // backward overwrites it (cleared by the previous optimizer step)
         model.backward(s, ctx, in, d_out.as<float>(), grads.p(), nullptr, GRAD_OVERWRITE, true);
         grad_stats(s, n, grads.p(), grad_partial.p(), bad_partial.p());
         train_finalize(s, hyper, loss_partial.p(), grad_partial.p(), bad_partial.p(), state.p());
+        // the next backward overwrites every gradient when the binned grid backward
+        // covers the tables; otherwise the grid part is accumulated into zeros
         adam_update(s, opt.adam, n, n_matrix, state.p(), params.p(), fp16 ? model.table16.p : nullptr, grads.p(),
-                    m1.p(), m2.p(), steps.p(), true);
+                    m1.p(), m2.p(), steps.p(), !model.grid_overwrite_covers_tables());
         if (opt.ema)
             ema_update(s, opt.ema_decay, n, state.p(), params.p(), ema.p(), fp16 ? model.table16_inference.p : nullptr);
         model.refresh_packed(s, true, opt.ema);
