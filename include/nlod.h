@@ -189,7 +189,8 @@ int nlod_model_encode_fragments(nlod_model* model, void* stream, uint32_t n, con
 typedef struct nlod_trainer nlod_trainer;
 
 enum nlod_trainer_flags {
-    NLOD_TRAINER_GRAPH = 1u << 0 /* capture each batch size's step in a hipGraph (after one eager step) */
+    NLOD_TRAINER_GRAPH = 1u << 0,  /* capture each batch size's step in a hipGraph (after one eager step) */
+    NLOD_TRAINER_UNFUSED = 1u << 1 /* FP16: separate forward / loss / backward kernels instead of the fused pass */
 };
 
 /* Trainer<float, T, T>(model, create_optimizer(json), create_loss(json), seed)

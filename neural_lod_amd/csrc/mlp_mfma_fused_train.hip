@@ -1,0 +1,436 @@
+// Fully fused FP16-MFMA training pass of the MLP (gfx950): forward, loss,
+// backward and weight gradients of one batch in one kernel.  Activations and
+// deltas never leave the CU; the weight gradients accumulate in MFMA
+// accumulators over the block's queries.  Semantics: forward as
+// tiny-cuda-nn fully_fused_mlp.cu:499-731, loss as the reference loss kernels,
+// backward as fully_fused_mlp.cu:150-259,733-833 (no biases).
+//
+// Block = 4 waves (one per SIMD, so a wave may use all 512 registers for its
+// activations and dW accumulators); a round = 4 tiles of 16 queries.  Per round:
+//   each wave: features X (B fragments) -> forward through the layers,
+//   keeping every activation as the next layer's B fragment in registers ->
+//   outputs -> loss value (FP64 partial) and dL/dy -> delta_out.
+//   For each layer from the output down: the wave computes its delta from
+//   the previous G and act'(H) in registers and stages delta and the layer's
+//   input (H or X) in LDS unit-major [unit][64 queries]; after a barrier every
+//   wave adds its share of the layer's dW tiles, dW += delta . input^T over the
+//   round's 64 queries (v_mfma_f32_16x16x32_f16, queries as the reduction
+//   dimension); then G = W^T delta for the next layer.
+//   Finally dL/d(enc) = W0^T delta_0 goes to global memory (FP32 SoA) for the
+//   encoding backward.
+// At the end each block writes its dW sums to a partial buffer; a second
+// kernel reduces the partials in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "device_common.h"
+#include "loss_device.h"
+#include "mfma_common.h"
+#include "model.h"
+
+namespace nlod {
+namespace {
+
+constexpr int kWaves = 4;  // one wave per SIMD: the whole register file (VGPRs + AGPRs) per wave
+constexpr int kThreads = kWaves * 64;
+constexpr int kQ = kWaves * 16;       // queries per round
+constexpr int kStride = kQ + 8;       // LDS row stride in halves (16 B aligned rows)
+constexpr int kMaxKTiles = 8;         // encoded width <= 128
+constexpr uint32_t kMaxFusedBlocks = 512;
+
+struct FusedArgs {
+    uint32_t N, n_output, k_enc, k_padded;
+    int32_t act, out_act, loss_type;
+    uint32_t fwd_halves, bwd_halves;
+    uint32_t n_net;
+};
+
+template <int W>
+struct Shape {
+    static constexpr int MT = W / 16;                        // 16-row tiles of a hidden layer
+    static constexpr int KC = W / 32;                        // 32-wide k chunks of a hidden layer
+    static constexpr int HT = (MT * MT + kWaves - 1) / kWaves;  // hidden dW tiles per wave
+    static constexpr int ZT = (MT * kMaxKTiles + kWaves - 1) / kWaves;  // layer-0 dW tiles per wave (max)
+};
+
+// dW tile (mr, nc) += A[16 mr + m][q] * B[16 nc + n][q] over the round's queries.
+__device__ __forceinline__ void dw_tile(const half_t* __restrict__ A, const half_t* __restrict__ B, int mr, int nc,
+                                        floatx4& acc) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, m = lane & 15;
+    const half_t* a = A + (size_t)(16 * mr + m) * kStride + 8 * g;
+    const half_t* b = B + (size_t)(16 * nc + m) * kStride + 8 * g;
+#pragma unroll
+    for (int q0 = 0; q0 < kQ; q0 += 32) {
+        const half8 av = *reinterpret_cast<const half8*>(a + q0);
+        const half8 bv = *reinterpret_cast<const half8*>(b + q0);
+        acc = mfma16(av, bv, acc);
+    }
+}
+
+// Stages a B-fragment-layout activation (units 32c+4g+i, 32c+16+4g+i of query n) as rows.
+template <int KC>
+__device__ __forceinline__ void stage_units(half_t* __restrict__ buf, const half8* v, int col) {
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            buf[(size_t)(32 * c + 4 * g + i) * kStride + col] = v[c][i];
+            buf[(size_t)(32 * c + 16 + 4 * g + i) * kStride + col] = v[c][i + 4];
+        }
+    }
+}
+
+template <int W, int L>
+__global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, const half_t* __restrict__ packed,
+                                                               const half_t* __restrict__ packed_bwd,
+                                                               const half_t* __restrict__ feat,
+                                                               const float* __restrict__ target,
+                                                               const float* __restrict__ loss_scale,
+                                                               double* __restrict__ loss_partial,
+                                                               float* __restrict__ d_enc,
+                                                               float* __restrict__ w_partial) {
+    using S = Shape<W>;
+    constexpr int MT = S::MT, KC = S::KC;
+    extern __shared__ float4 smem4[];
+    half_t* wf = reinterpret_cast<half_t*>(smem4);
+    half_t* wb = wf + a.fwd_halves;
+    half_t* bufA = wb + a.bwd_halves;                 // delta rows [W][kStride]
+    half_t* bufB = bufA + (size_t)W * kStride;        // input rows [ROWS_B][kStride]
+    __shared__ double red[kWaves];
+    for (uint32_t k = threadIdx.x; k < a.fwd_halves / 8; k += kThreads)
+        reinterpret_cast<float4*>(wf)[k] = reinterpret_cast<const float4*>(packed)[k];
+    for (uint32_t k = threadIdx.x; k < a.bwd_halves / 8; k += kThreads)
+        reinterpret_cast<float4*>(wb)[k] = reinterpret_cast<const float4*>(packed_bwd)[k];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15, wave = threadIdx.x >> 6;
+    const uint32_t kc0 = a.k_padded / 32, kp8 = a.k_padded / 8;
+    const uint32_t KT = (a.k_enc + 15) / 16;  // layer-0 dW column tiles
+    const uint32_t n_tiles = (a.N + 15) / 16;
+    const uint32_t rounds = (n_tiles + gridDim.x * kWaves - 1) / (gridDim.x * kWaves);
+    const float scale = *loss_scale;
+    const float n_total = (float)((uint64_t)a.N * a.n_output);
+
+    floatx4 acc_out = {0.f, 0.f, 0.f, 0.f};
+    floatx4 acc_hid[L > 1 ? L - 1 : 1][S::HT];
+    floatx4 acc_in[S::ZT];
+#pragma unroll
+    for (int l = 0; l < (L > 1 ? L - 1 : 1); ++l)
+#pragma unroll
+        for (int t = 0; t < S::HT; ++t) acc_hid[l][t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < S::ZT; ++t) acc_in[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    double lsum = 0.0;
+
+    // weight-fragment offsets (halves) of the forward and backward packings
+    const size_t f_hidden0 = (size_t)MT * kc0 * 512;
+    const size_t f_out = f_hidden0 + (size_t)(L - 1) * MT * KC * 512;
+    const size_t b_hidden0 = (size_t)MT * 512;
+    const size_t b_in = b_hidden0 + (size_t)(L - 1) * MT * KC * 512;
+
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint32_t tile = (r * gridDim.x + blockIdx.x) * kWaves + wave;
+        const bool tile_ok = tile < n_tiles;
+        const uint32_t q = tile * 16 + n;
+        const bool valid = tile_ok && q < a.N;
+        const int col = wave * 16 + n;
+
+        // ---- forward
+        half8 xb[4];  // k_padded <= 128: 4 chunks of 32 features
+        floatx4 acc[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+        const half_t* fb = feat + (size_t)(tile_ok ? tile : 0) * kp8 * 128;
+#pragma unroll
+        for (uint32_t c = 0; c < 4; ++c) {
+            if (c >= kc0) break;
+            xb[c] = tile_ok ? *reinterpret_cast<const half8*>(fb + ((size_t)(4 * c + g) * 16 + n) * 8) : half8{};
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const half8 w = *reinterpret_cast<const half8*>(wf + ((size_t)(mt * kc0 + c) * 64 + lane) * 8);
+                acc[mt] = mfma16(w, xb[c], acc[mt]);
+            }
+        }
+        half8 h[L][KC];
+#pragma unroll
+        for (int c = 0; c < KC; ++c) h[0][c] = pack_act(acc[2 * c], acc[2 * c + 1], a.act);
+#pragma unroll
+        for (int l = 1; l < L; ++l) {
+            const half_t* fr = wf + f_hidden0 + (size_t)(l - 1) * MT * KC * 512;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                floatx4 s = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int c = 0; c < KC; ++c) {
+                    const half8 w = *reinterpret_cast<const half8*>(fr + ((size_t)(mt * KC + c) * 64 + lane) * 8);
+                    s = mfma16(w, h[l - 1][c], s);
+                }
+                acc[mt] = s;
+            }
+#pragma unroll
+            for (int c = 0; c < KC; ++c) h[l][c] = pack_act(acc[2 * c], acc[2 * c + 1], a.act);
+        }
+        floatx4 o = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            const half8 w = *reinterpret_cast<const half8*>(wf + f_out + ((size_t)c * 64 + lane) * 8);
+            o = mfma16(w, h[L - 1][c], o);
+        }
+
+        // ---- loss and delta_out (rows 4g+i = outputs of query n)
+        half8 bo;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t oi = 4 * g + i;
+            float d = 0.0f;
+            if (valid && oi < a.n_output) {
+                const float p = act_fwd(a.out_act, o[i]);
+                double v;
+                float gr;
+                loss_elem(a.loss_type, p, target[(size_t)q * a.n_output + oi], n_total, scale, v, gr);
+                lsum += v;
+                d = gr * act_bwd(a.out_act, p);
+            }
+            bo[i] = (half_t)d;
+            bo[i + 4] = (half_t)0.0f;
+        }
+
+        // ---- output layer: dW_out = delta_out . H_{L-1}^T
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bufA[(size_t)(4 * g + i) * kStride + col] = bo[i];
+        stage_units<KC>(bufB, h[L - 1], col);
+        __syncthreads();
+        if (wave < MT) dw_tile(bufA, bufB, 0, wave, acc_out);
+        __syncthreads();
+        // G = W_out^T delta_out
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const half8 w = *reinterpret_cast<const half8*>(wb + ((size_t)mt * 64 + lane) * 8);
+            acc[mt] = mfma16(w, bo, floatx4{0.f, 0.f, 0.f, 0.f});
+        }
+
+        // ---- hidden layers L-1 .. 0
+        half8 dl[KC];
+#pragma unroll
+        for (int l = L - 1; l >= 0; --l) {
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    dl[c][i] = (half_t)(acc[2 * c][i] * act_bwd(a.act, (float)h[l][c][i]));
+                    dl[c][i + 4] = (half_t)(acc[2 * c + 1][i] * act_bwd(a.act, (float)h[l][c][i + 4]));
+                }
+            }
+            stage_units<KC>(bufA, dl, col);
+            if (l > 0) {
+                stage_units<KC>(bufB, h[l - 1], col);
+            } else {
+                // layer 0 input: the features, natural order (feature 32c + 8g + j of query n)
+#pragma unroll
+                for (uint32_t c = 0; c < 4; ++c) {
+                    if (c >= kc0) break;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) bufB[(size_t)(32 * c + 8 * g + j) * kStride + col] = xb[c][j];
+                }
+            }
+            __syncthreads();
+            if (l > 0) {
+#pragma unroll
+                for (int t = 0; t < S::HT; ++t) {
+                    const int ti = wave + kWaves * t;
+                    if (ti < MT * MT) dw_tile(bufA, bufB, ti / MT, ti % MT, acc_hid[l - 1][t]);
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < S::ZT; ++t) {
+                    const int ti = wave + kWaves * t;
+                    if (ti < (int)(MT * KT)) dw_tile(bufA, bufB, ti / (int)KT, ti % (int)KT, acc_in[t]);
+                }
+            }
+            __syncthreads();
+            if (l > 0) {
+                // G = W_l^T delta_l
+                const half_t* fr = wb + b_hidden0 + (size_t)(L - 1 - l) * MT * KC * 512;
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    floatx4 s = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int c = 0; c < KC; ++c) {
+                        const half8 w = *reinterpret_cast<const half8*>(fr + ((size_t)(mt * KC + c) * 64 + lane) * 8);
+                        s = mfma16(w, dl[c], s);
+                    }
+                    acc[mt] = s;
+                }
+            }
+        }
+
+        // ---- dL/d(enc) = W0^T delta_0, rows = encoded features
+        if (d_enc) {
+            const uint32_t mt0 = a.k_padded / 16;
+            for (uint32_t mt = 0; mt < mt0; ++mt) {
+                floatx4 s = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int c = 0; c < KC; ++c) {
+                    const half8 w = *reinterpret_cast<const half8*>(wb + b_in + ((size_t)(mt * KC + c) * 64 + lane) * 8);
+                    s = mfma16(w, dl[c], s);
+                }
+                if (valid) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const uint32_t row = 16 * mt + 4 * g + i;
+                        if (row < a.k_enc) d_enc[(size_t)row * a.N + q] = s[i];
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- per-block results: loss partial and dW partial (row-major matrices)
+#pragma unroll
+    for (int o2 = 32; o2 > 0; o2 >>= 1) lsum += __shfl_down(lsum, o2, 64);
+    if (lane == 0) red[wave] = lsum;
+    float* wp = w_partial + (size_t)blockIdx.x * a.n_net;
+    const size_t off_hidden = (size_t)W * a.k_enc;
+    const size_t off_out = off_hidden + (size_t)(L - 1) * W * W;
+    // output layer: rows = outputs (16 padded), cols = W
+    if (wave < MT) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) wp[off_out + (size_t)(4 * g + i) * W + 16 * wave + n] = acc_out[i];
+    }
+#pragma unroll
+    for (int l = 1; l < L; ++l) {
+#pragma unroll
+        for (int t = 0; t < S::HT; ++t) {
+            const int ti = wave + kWaves * t;
+            if (ti >= MT * MT) continue;
+            const int mr = ti / MT, nc = ti % MT;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                wp[off_hidden + (size_t)(l - 1) * W * W + (size_t)(16 * mr + 4 * g + i) * W + 16 * nc + n] =
+                    acc_hid[l - 1][t][i];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < S::ZT; ++t) {
+        const int ti = wave + kWaves * t;
+        if (ti >= (int)(MT * KT)) continue;
+        const int mr = ti / (int)KT, nc = ti % (int)KT;
+        const uint32_t colk = 16 * nc + n;
+        if (colk < a.k_enc) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) wp[(size_t)(16 * mr + 4 * g + i) * a.k_enc + colk] = acc_in[t][i];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < kWaves; ++w) s += red[w];
+        loss_partial[blockIdx.x] = s;
+    }
+}
+
+// out[e] (+)= sum over partial rows, in row order; 8 rows in flight per thread.
+__global__ __launch_bounds__(256) void partial_reduce_kernel(uint32_t n, uint32_t rows,
+                                                             const float* __restrict__ partial,
+                                                             float* __restrict__ out, int accumulate) {
+    const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    float s = 0.0f;
+    uint32_t r = 0;
+    for (; r + 8 <= rows; r += 8) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = partial[(size_t)(r + k) * n + e];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += v[k];
+    }
+    for (; r < rows; ++r) s += partial[(size_t)r * n + e];
+    out[e] = accumulate ? out[e] + s : s;
+}
+
+template <typename Fn>
+void dispatch_wl(uint32_t W, uint32_t L, Fn&& f) {
+    auto with_l = [&](auto Wc) {
+        switch (L) {
+            case 1: f(Wc, std::integral_constant<int, 1>{}); break;
+            case 2: f(Wc, std::integral_constant<int, 2>{}); break;
+            case 3: f(Wc, std::integral_constant<int, 3>{}); break;
+            case 4: f(Wc, std::integral_constant<int, 4>{}); break;
+            case 5: f(Wc, std::integral_constant<int, 5>{}); break;
+            case 6: f(Wc, std::integral_constant<int, 6>{}); break;
+            case 7: f(Wc, std::integral_constant<int, 7>{}); break;
+            case 8: f(Wc, std::integral_constant<int, 8>{}); break;
+            default: fail(NLOD_ERR_UNSUPPORTED, "fused training supports 1-8 hidden layers");
+        }
+    };
+    switch (W) {
+        case 32: with_l(std::integral_constant<int, 32>{}); break;
+        case 64: with_l(std::integral_constant<int, 64>{}); break;
+        default: fail(NLOD_ERR_UNSUPPORTED, "fused training supports widths 32 and 64");
+    }
+}
+
+size_t fused_smem(const Layout& L) {
+    const uint32_t W = L.net.width;
+    const uint32_t rows_b = std::max<uint32_t>(W, k_padded_of(L.net));
+    return mfma_packed_bytes(L.net) + mfma_packed_bwd_bytes(L.net) + sizeof(half_t) * (size_t)(W + rows_b) * kStride;
+}
+
+}  // namespace
+
+bool mfma_fused_train_supported(const Layout& L) {
+    const NetShape& n = L.net;
+    if (n.width != 32 && n.width != 64) return false;
+    // register budget of one wave: activations of every layer plus its dW tiles
+    if (n.n_hidden_layers < 1 || n.n_hidden_layers > (n.width == 64 ? 5u : 8u)) return false;
+    if (k_padded_of(n) > 16 * kMaxKTiles) return false;
+    return fused_smem(L) <= 160 * 1024;
+}
+
+uint32_t mfma_fused_train_blocks(const Layout& L, uint32_t N) {
+    const uint32_t rounds_min = div_round_up(div_round_up(std::max<uint32_t>(N, 1), 16), kWaves);
+    return std::max<uint32_t>(1, std::min<uint32_t>({rounds_min, (uint32_t)cu_count(), kMaxFusedBlocks}));
+}
+
+void mfma_fused_train(hipStream_t stream, const Layout& L, const void* packed, const void* packed_bwd,
+                      const void* feat, const float* target, uint32_t N, int loss_type, const float* loss_scale,
+                      double* loss_partial, float* d_enc, float* w_partial) {
+    if (!mfma_fused_train_supported(L)) fail(NLOD_ERR_UNSUPPORTED, "layout not supported by the fused training kernel");
+    FusedArgs a{};
+    a.N = N;
+    a.n_output = L.net.n_output;
+    a.k_enc = L.net.n_input;
+    a.k_padded = k_padded_of(L.net);
+    a.act = L.net.activation;
+    a.out_act = L.net.output_activation;
+    a.loss_type = loss_type;
+    a.fwd_halves = (uint32_t)(mfma_packed_bytes(L.net) / sizeof(half_t));
+    a.bwd_halves = (uint32_t)(mfma_packed_bwd_bytes(L.net) / sizeof(half_t));
+    a.n_net = (uint32_t)L.net.n_params();
+    const size_t smem = fused_smem(L);
+    const uint32_t blocks = mfma_fused_train_blocks(L, N);
+    dispatch_wl(L.net.width, L.net.n_hidden_layers, [&](auto Wc, auto Lc) {
+        constexpr int W = decltype(Wc)::value, LL = decltype(Lc)::value;
+        if constexpr (W == 64 && LL > 5) {
+            fail(NLOD_ERR_UNSUPPORTED, "fused training supports at most 5 hidden layers at width 64");
+        } else {
+            auto kern = fused_train_kernel<W, LL>;
+            (void)occupancy_blocks((const void*)kern, kThreads, smem);  // raises the dynamic LDS limit
+            hipLaunchKernelGGL(kern, dim3(blocks), dim3(kThreads), smem, stream, a,
+                               static_cast<const half_t*>(packed), static_cast<const half_t*>(packed_bwd),
+                               static_cast<const half_t*>(feat), target, loss_scale, loss_partial, d_enc, w_partial);
+        }
+    });
+    NLOD_LAUNCH_CHECK();
+}
+
+void partial_reduce(hipStream_t stream, uint32_t n, uint32_t rows, const float* partial, float* out, bool accumulate) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(partial_reduce_kernel, dim3(div_round_up(n, 256)), dim3(256), 0, stream, n, rows, partial, out,
+                       accumulate ? 1 : 0);
+    NLOD_LAUNCH_CHECK();
+}
+
+}  // namespace nlod

@@ -222,6 +222,24 @@ void Model::backward(hipStream_t s, ForwardCtx& ctx, const float* in, const floa
     }
 }
 
+uint32_t Model::train_fused(hipStream_t s, uint32_t N, const float* in, const float* target, int loss_type,
+                           const float* loss_scale, double* loss_partial, uint32_t max_partials, float* d_params) {
+    if (precision != PREC_FP16 || N == 0 || N > kRecomputeAbove || !mfma_fused_train_supported(L)) return 0;
+    const uint32_t blocks = mfma_fused_train_blocks(L, N);
+    if (blocks > max_partials) return 0;
+    require_params(false);
+    const bool has_grid = L.n_params > L.n_net_params;
+    ws_feat.ensure(mfma_feature_bytes(L, N));
+    if (has_grid) ws_denc.ensure(sizeof(float) * (size_t)L.enc_width * N);
+    ws_wgrad.ensure(sizeof(float) * (size_t)blocks * L.n_net_params);
+    mfma_encode(s, L, program.p, table_half(false), in, N, ws_feat.p);
+    mfma_fused_train(s, L, packed_fwd(false), packed_back(false), ws_feat.p, target, N, loss_type, loss_scale,
+                     loss_partial, has_grid ? ws_denc.as<float>() : nullptr, ws_wgrad.as<float>());
+    partial_reduce(s, (uint32_t)L.n_net_params, blocks, ws_wgrad.as<float>(), d_params, false);
+    if (has_grid) grid_backward(s, in, N, ws_denc.as<float>(), d_params, true);
+    return blocks;
+}
+
 // Parameter initialisation of tiny-cuda-nn's Trainer (trainer.h:53-92):
 // pcg32 seeded from std::seed_seq{seed}; network matrices Xavier-uniform on the
 // host in order (gpu_matrix.h:291-306), then each grid table U[-1e-4, 1e-4]

@@ -94,6 +94,12 @@ struct Model {
                        bool overwrite);
     // true when grid_backward(overwrite) writes every table entry itself
     bool grid_overwrite_covers_tables() const { return grid_bwd && grid_bwd->supported(); }
+    // FP16 form: one training pass (encoding, fused MLP forward + loss + backward +
+    // dW, grid backward) writing every gradient (overwrite) and the FP64 loss
+    // partials; returns the number of loss partials written (0: not available
+    // for this layout, nothing was done)
+    uint32_t train_fused(hipStream_t s, uint32_t N, const float* in, const float* target, int loss_type,
+                         const float* loss_scale, double* loss_partial, uint32_t max_partials, float* d_params);
 };
 
 // Batches above this size recompute their activations chunk by chunk in the
@@ -137,5 +143,17 @@ size_t mfma_wgrad_workspace_floats(const Layout& L, uint32_t N);
 // network weight gradients (FP32, deterministic) written or added into grads[0 .. n_network_params)
 void mfma_weight_grad(hipStream_t stream, const Layout& L, const void* feat, const void* hidden, const void* delta,
                       uint32_t N, float* grads, float* workspace, size_t workspace_floats, bool accumulate);
+
+
+// ---- mlp_mfma_fused_train.hip (forward + loss + backward + dW in one kernel) -------
+bool mfma_fused_train_supported(const Layout& L);
+// blocks of the fused kernel = loss partials and dW partial rows it writes
+uint32_t mfma_fused_train_blocks(const Layout& L, uint32_t N);
+// loss_partial[blocks] (FP64), d_enc SoA float [K][N] (nullable), w_partial [blocks][n_network_params]
+void mfma_fused_train(hipStream_t stream, const Layout& L, const void* packed, const void* packed_bwd,
+                      const void* feat, const float* target, uint32_t N, int loss_type, const float* loss_scale,
+                      double* loss_partial, float* d_enc, float* w_partial);
+// out[e] (+)= sum over rows of partial[rows][n], in row order
+void partial_reduce(hipStream_t stream, uint32_t n, uint32_t rows, const float* partial, float* out, bool accumulate);
 
 }  // namespace nlod

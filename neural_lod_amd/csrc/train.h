@@ -55,8 +55,8 @@ void loss_evaluate(hipStream_t s, int type, uint64_t n_elems, float scale, const
                    float* values, float* grads);
 // FP64 partial sums of g^2 and non-finite counts [kGradPartials]
 void grad_stats(hipStream_t s, uint64_t n, const float* g, double* partial_sq, uint32_t* partial_bad);
-void train_finalize(hipStream_t s, const TrainHyper& h, const double* loss_partial, const double* grad_partial,
-                    const uint32_t* bad_partial, TrainState* st);
+void train_finalize(hipStream_t s, const TrainHyper& h, uint32_t n_loss_partials, const double* loss_partial,
+                    const double* grad_partial, const uint32_t* bad_partial, TrainState* st);
 // Adam over n parameters (the first n_matrix are matrix weights); w_half (nullable)
 // receives an FP16 copy of every updated weight; clear_grads zeroes consumed gradients.
 void adam_update(hipStream_t s, const AdamParams& a, uint64_t n, uint64_t n_matrix, const TrainState* st, float* w,

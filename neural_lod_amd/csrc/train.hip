@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdint>
 
+#include "loss_device.h"
 #include "train.h"
 
 namespace nlod {
@@ -31,30 +32,6 @@ __device__ __forceinline__ T block_sum(T v, T* red) {
         for (int w = 0; w < kBlock / 64; ++w) s += red[w];
     __syncthreads();
     return s;  // valid in thread 0
-}
-
-// Per-element value and scaled gradient, as the reference's loss kernels
-// (losses/binary_cross_entropy.h:7-43, losses/relative_l2.h:39-76,
-// losses/l2.h) for one prediction p and target t; n_total = N * dims.
-__device__ __forceinline__ void loss_elem(int type, float p, float t, float n_total, float scale, double& value,
-                                          float& grad) {
-    if (type == LOSS_BCE) {
-        const float eps = 1e-5f;
-        const float pc = fminf(fmaxf(p, eps), 1.0f - eps);
-        // as the reference evaluates it: t*log(p) in float, the (1.0 - t) term in double
-        value = -((double)(t * logf(pc)) + (1.0 - (double)t) * log(1.0 - (double)pc)) / (double)n_total;
-        const float g = -(t - pc) / (pc - pc * pc);
-        grad = scale * g / n_total;
-    } else if (type == LOSS_RELATIVE_L2) {
-        const float den = p * p + 0.01f;
-        const float d = p - t;
-        value = (double)(d * d / den / n_total);
-        grad = scale * (2.0f * d / den) / n_total;
-    } else {  // L2
-        const float d = p - t;
-        value = (double)(d * d / n_total);
-        grad = scale * (2.0f * d) / n_total;
-    }
 }
 
 // d_out = loss_scale * dL/dpred (float [n][dims]); per-block FP64 partial sums of the loss.
@@ -324,9 +301,9 @@ void grad_stats(hipStream_t s, uint64_t n, const float* g, double* partial_sq, u
     NLOD_LAUNCH_CHECK();
 }
 
-void train_finalize(hipStream_t s, const TrainHyper& h, const double* loss_partial, const double* grad_partial,
-                    const uint32_t* bad_partial, TrainState* st) {
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kBlock), 0, s, h, (uint32_t)kLossPartials, loss_partial,
+void train_finalize(hipStream_t s, const TrainHyper& h, uint32_t n_loss_partials, const double* loss_partial,
+                    const double* grad_partial, const uint32_t* bad_partial, TrainState* st) {
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(kBlock), 0, s, h, n_loss_partials, loss_partial,
                        (uint32_t)kGradPartials, grad_partial, bad_partial, st);
     NLOD_LAUNCH_CHECK();
 }

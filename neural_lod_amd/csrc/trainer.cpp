@@ -97,6 +97,7 @@ struct Trainer {
     TrainHyper hyper{};
     float initial_scale = 1024.0f;
     bool use_graphs = false;
+    bool fused = true;  // FP16: fused forward + loss + backward kernel where the layout allows
     uint64_t n = 0, n_matrix = 0;
 
     DevArray<float> params, grads, m1, m2, ema;
@@ -112,8 +113,9 @@ struct Trainer {
     std::map<uint32_t, hipGraphExec_t> graphs;
     std::map<uint32_t, int> warm;  // batch sizes that ran once eagerly (workspaces sized)
 
-    Trainer(Model& m, const Json& optimizer, const Json& loss, const Json& scaler, uint32_t seed, bool graphs_on)
-        : model(m), use_graphs(graphs_on) {
+    Trainer(Model& m, const Json& optimizer, const Json& loss, const Json& scaler, uint32_t seed, bool graphs_on,
+            bool fused_on)
+        : model(m), use_graphs(graphs_on), fused(fused_on) {
         parse_optimizer(optimizer, opt);
         loss_type = parse_loss(loss);
         initial_scale = (float)scaler.value("initial_scale", 1024.0);
@@ -186,15 +188,22 @@ struct Trainer {
 
     void step_body(hipStream_t s, uint32_t N, const float* in, const float* target) {
         const bool fp16 = model.precision == PREC_FP16;
-        model.forward_into(s, N, in, nullptr, false, false, ctx);
-        const uint64_t n_out = (uint64_t)N * model.L.net.n_output;
-        loss_forward(s, loss_type, n_out, model.L.net.n_output, state.p(), ctx.out.as<float>(), target,
-                     d_out.as<float>(), loss_partial.p());
-        // the grid part of the gradient buffer is all zero here unless the grid
-        // backward overwrites it (cleared by the previous optimizer step)
-        model.backward(s, ctx, in, d_out.as<float>(), grads.p(), nullptr, GRAD_OVERWRITE, true);
+        // FP16 form: one fused pass writes every gradient and the loss partials
+        uint32_t n_loss = fused ? model.train_fused(s, N, in, target, loss_type, &state.p()->loss_scale,
+                                                    loss_partial.p(), kLossPartials, grads.p())
+                                : 0;
+        if (n_loss == 0) {
+            model.forward_into(s, N, in, nullptr, false, false, ctx);
+            const uint64_t n_out = (uint64_t)N * model.L.net.n_output;
+            loss_forward(s, loss_type, n_out, model.L.net.n_output, state.p(), ctx.out.as<float>(), target,
+                         d_out.as<float>(), loss_partial.p());
+            n_loss = kLossPartials;
+            // the grid part of the gradient buffer is all zero here unless the grid
+            // backward overwrites it (cleared by the previous optimizer step)
+            model.backward(s, ctx, in, d_out.as<float>(), grads.p(), nullptr, GRAD_OVERWRITE, true);
+        }
         grad_stats(s, n, grads.p(), grad_partial.p(), bad_partial.p());
-        train_finalize(s, hyper, loss_partial.p(), grad_partial.p(), bad_partial.p(), state.p());
+        train_finalize(s, hyper, n_loss, loss_partial.p(), grad_partial.p(), bad_partial.p(), state.p());
         // the next backward overwrites every gradient when the binned grid backward
         // covers the tables; otherwise the grid part is accumulated into zeros
         adam_update(s, opt.adam, n, n_matrix, state.p(), params.p(), fp16 ? model.table16.p : nullptr, grads.p(),
@@ -350,12 +359,13 @@ int nlod_trainer_create(nlod_model* model, const char* optimizer_json, const cha
     return nlod::guarded([&] {
         NLOD_CHECK(model && out && optimizer_json && loss_json, "null argument");
         *out = nullptr;
-        NLOD_CHECK((flags & ~1u) == 0, "unknown trainer flags");
+        NLOD_CHECK((flags & ~3u) == 0, "unknown trainer flags");
         const nlod::Json opt = nlod::Json::parse(optimizer_json);
         const nlod::Json loss = nlod::Json::parse(loss_json);
         const nlod::Json scaler = nlod::Json::parse(loss_scale_json ? loss_scale_json : "{}");
         auto h = std::make_unique<nlod_trainer>();
-        h->t = std::make_unique<nlod::Trainer>(nlod::model_of(model), opt, loss, scaler, seed, (flags & 1u) != 0);
+        h->t = std::make_unique<nlod::Trainer>(nlod::model_of(model), opt, loss, scaler, seed, (flags & 1u) != 0,
+                                               (flags & 2u) == 0);
         *out = h.release();
     });
 }

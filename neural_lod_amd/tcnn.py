@@ -185,6 +185,7 @@ class NetworkWithInputEncoding:
 
 LOSS_L2, LOSS_RELATIVE_L2, LOSS_BCE = 0, 1, 2
 TRAINER_GRAPH = 1
+TRAINER_UNFUSED = 2
 
 
 class Trainer:
@@ -194,11 +195,11 @@ class Trainer:
     parameters are bound to the trainer's buffers while the trainer lives."""
 
     def __init__(self, model: NetworkWithInputEncoding, optimizer_config, loss_config, seed: int = 1337,
-                 loss_scale_config=None, graphs: bool = False):
+                 loss_scale_config=None, graphs: bool = False, fused: bool = True):
         h = C.c_void_p()
+        flags = (TRAINER_GRAPH if graphs else 0) | (0 if fused else TRAINER_UNFUSED)
         _lib.call("nlod_trainer_create", model._h, _as_json(optimizer_config), _as_json(loss_config),
-                  _as_json(loss_scale_config) if loss_scale_config is not None else None, seed,
-                  TRAINER_GRAPH if graphs else 0, C.byref(h))
+                  _as_json(loss_scale_config) if loss_scale_config is not None else None, seed, flags, C.byref(h))
         self._h = h
         self.model = model
         model._params = None

@@ -37,10 +37,10 @@ def _model(cfg, precision):
                                     precision)
 
 
-def _trainer(cfg, precision, optimizer=ADAM, loss=BCE, seed=1337, scale=None, graphs=False):
+def _trainer(cfg, precision, optimizer=ADAM, loss=BCE, seed=1337, scale=None, graphs=False, fused=True):
     from neural_lod_amd.tcnn import Trainer
     m = _model(cfg, precision)
-    return Trainer(m, optimizer, loss, seed=seed, loss_scale_config=scale, graphs=graphs)
+    return Trainer(m, optimizer, loss, seed=seed, loss_scale_config=scale, graphs=graphs, fused=fused)
 
 
 def _batches(n, count, seed):
@@ -193,6 +193,40 @@ def test_trainer_graph_replay_matches_eager(cuda, precision):
     pa, pb = eager.params(), graph.params()
     close = (pa - pb).abs() <= 1e-5 + 1e-4 * pa.abs()
     assert close.float().mean().item() >= 0.999
+
+
+@pytest.mark.parametrize("name", ["visibility", "throughput", "small_vis"])
+def test_fused_training_pass_matches_unfused(cuda, name):
+    """FP16: the fused forward + loss + backward + dW kernel gives the loss and
+    gradients of the separate kernels (same FP16 rounding points; dW summed in
+    another order)."""
+    cfg = SMALL_VIS if name == "small_vis" else configs.ALL[name]
+    n = 3000
+    if cfg["n_input_dims"] == 6:
+        x = synthetic.visibility_queries(n, 90)
+        t = synthetic.sphere_visibility_labels(x)
+        loss = BCE
+    else:
+        x = synthetic.throughput_queries(n, seed=91)
+        t = np.stack([0.3 + 0.2 * np.sin(4 * x[:, 0]), 0.5 + 0.1 * x[:, 1], 0.2 + 0.3 * x[:, 2]], 1).astype(np.float32)
+        loss = {"otype": "RelativeL2"}
+    scale = {"initial_scale": 1024.0, "dynamic": False}
+    tf = _trainer(cfg, 1, loss=loss, seed=5, scale=scale)
+    tu = _trainer(cfg, 1, loss=loss, seed=5, scale=scale, fused=False)
+    xt, tt = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
+    tf.training_step(xt, tt)
+    tu.training_step(xt, tt)
+    lf, lu = tf.loss(), tu.loss()
+    assert lf == pytest.approx(lu, rel=1e-6), (lf, lu)
+    gf, gu = tf.gradients().cpu().numpy().astype(np.float64), tu.gradients().cpu().numpy().astype(np.float64)
+    L = M.Layout(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"])
+    from test_backward_fp16_gpu import _blocks
+    for blk, sl in _blocks(L):
+        rel = np.linalg.norm(gf[sl] - gu[sl]) / max(np.linalg.norm(gu[sl]), 1e-30)
+        print(f"  {blk}: fused vs unfused rel L2 {rel:.2e}")
+        assert rel <= 1e-4, (blk, rel)
+    assert torch.equal(tf.params(), tu.params()) or \
+        ((tf.params() - tu.params()).abs() > 1e-6).float().mean().item() < 1e-3
 
 
 def test_snapshot_roundtrip(cuda):
