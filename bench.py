@@ -49,6 +49,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-train", action="store_true", help="skip the training-step measurement")
     ap.add_argument("--train-steps", type=int, default=50)
+    ap.add_argument("--profile-encode", action="store_true",
+                    help="only run the FP16 encoding pass and a 256 MiB device copy (for counter profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); run with --gpus 1", file=sys.stderr)
@@ -68,6 +70,22 @@ def main():
     params = torch.from_numpy(params_np).to(dev)
     x = torch.from_numpy(x_np).to(dev)
     log(f"workload: {N_QUERIES} queries, {L.n_params} params ({L.n_params * 4 / 2**20:.1f} MiB FP32)")
+
+    if args.profile_encode:
+        # the dominant inference kernel alone, then a plain copy of known size that
+        # calibrates the HBM byte counters (FETCH_SIZE / WRITE_SIZE)
+        m = NetworkWithInputEncoding(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"], FP16)
+        m.set_params(params)
+        feat = m.encode_fragments(x)
+        for _ in range(args.steps):
+            m.encode_fragments(x, feat)
+        src = torch.empty(1 << 26, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"profile-encode: {args.steps} encoding passes of {N_QUERIES} queries, {args.steps} copies of 256 MiB")
+        return
 
     models = {}
     for name, prec in (("fp16", FP16), ("fp32", FP32)):

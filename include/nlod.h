@@ -177,8 +177,9 @@ int nlod_model_encode(nlod_model* model, void* stream, uint32_t n, const float* 
 int nlod_model_encode_backward(nlod_model* model, void* stream, uint32_t n, const float* d_in,
                                const float* d_dL_dencoded, float* d_dL_dparams, int grad_mode);
 
-/* FP16 models: the encoding pass alone, writing FP16 features in the MFMA
- * fragment layout [ceil(n/16)][K/8][16][8] (K = encoded width padded to 32),
+/* FP16 models: the encoding pass alone, writing FP16 features in the layout
+ * the MFMA kernels read, [ceil(n/16)][K/2][16][2] (16-query tiles of feature
+ * pairs; K = encoded width padded to 32),
  * n_bytes from nlod_model_feature_bytes.  For profiling and for callers that
  * reuse one encoding across several networks; no counterpart in the reference. */
 int nlod_model_feature_bytes(const nlod_model* model, uint32_t n, uint64_t* bytes_out);

@@ -35,6 +35,23 @@ struct EncProgram {
 
 inline uint32_t k_padded_of(const NetShape& n) { return (n.n_input + 31) / 32 * 32; }
 
+// FP16 encoded features of a batch: [N/16 tiles][K/2 feature pairs][16 queries][2 halves].
+// A grid level with F = 2 fills one contiguous 64-byte row of a tile, so the
+// per-level encoding pass writes whole cache lines.  Offset in halves:
+__host__ __device__ inline size_t feat_pair_offset(uint32_t tile, uint32_t pair, uint32_t kp2, uint32_t n) {
+    return (((size_t)tile * kp2 + pair) * 16 + n) * 2;
+}
+
+// B fragment of k-chunk c for lane group g and query n of a tile: features
+// 32c + 8g .. 32c + 8g + 7, i.e. pairs 16c + 4g .. 16c + 4g + 3 (4 dwords, 64 B apart).
+__device__ __forceinline__ half8 load_feat_frag(const half_t* __restrict__ feat, uint32_t tile, uint32_t kp2,
+                                                uint32_t c, uint32_t g, uint32_t n) {
+    const uint32_t* p = reinterpret_cast<const uint32_t*>(feat) + ((size_t)tile * kp2 + 16 * c + 4 * g) * 16 + n;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 w = {p[0], p[16], p[32], p[48]};
+    return __builtin_bit_cast(half8, w);
+}
+
 // Packed FP16 weight fragments: per matrix, per 16-row tile, per 32-column
 // chunk, 64 lanes x 8 halves (layer 0 in natural column order, later layers
 // with columns permuted to the accumulator layout).
@@ -121,6 +138,86 @@ __device__ __forceinline__ void grid_level(const GridLevels& lv, uint32_t level,
         const uint32_t idx = grid_index(lv.grid_type, hs, res, cell[0] + (c & 1), cell[1] + ((c >> 1) & 1),
                                         cell[2] + ((c >> 2) & 1));
         load_entry<F, TT>(tab + (size_t)idx * F, vals[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        float w = 1.0f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) w *= ((c >> d) & 1) ? fr[d] : 1.0f - fr[d];
+#pragma unroll
+        for (int f = 0; f < F; ++f) v[f] += w * vals[c][f];
+    }
+}
+
+// Entry e (0 <= e < 8 / F) of a 16-byte block of FP16 entries with F features.
+template <int F>
+__device__ __forceinline__ void entry_of_block(const uint4& blk, uint32_t e, float* out) {
+    const uint32_t w[4] = {blk.x, blk.y, blk.z, blk.w};
+    if constexpr (F == 8) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            out[2 * k] = (float)__builtin_bit_cast(half_t, (uint16_t)(w[k] & 0xffffu));
+            out[2 * k + 1] = (float)__builtin_bit_cast(half_t, (uint16_t)(w[k] >> 16));
+        }
+    } else {
+        constexpr uint32_t WORDS = F == 1 ? 1 : F / 2;  // 32-bit words per entry (F = 1: half a word)
+        const uint32_t word0 = F == 1 ? e / 2 : e * WORDS;
+        uint32_t sel[WORDS];
+#pragma unroll
+        for (uint32_t k = 0; k < WORDS; ++k) {
+            const uint32_t wi = word0 + k;
+            sel[k] = wi == 0 ? w[0] : wi == 1 ? w[1] : wi == 2 ? w[2] : w[3];
+        }
+        if constexpr (F == 1) {
+            const uint32_t bits = (e & 1) ? (sel[0] >> 16) : (sel[0] & 0xffffu);
+            out[0] = (float)__builtin_bit_cast(half_t, (uint16_t)bits);
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < WORDS; ++k) {
+                out[2 * k] = (float)__builtin_bit_cast(half_t, (uint16_t)(sel[k] & 0xffffu));
+                out[2 * k + 1] = (float)__builtin_bit_cast(half_t, (uint16_t)(sel[k] >> 16));
+            }
+        }
+    }
+}
+
+// One level of a grid for one query from an FP16 table, F features in FP32.
+// The two corners of an x-edge usually share a 16-byte block of the table
+// (dense levels: adjacent entries; coherent prime hash: x enters with prime 1,
+// so the entries differ only in the low bits x0 ^ (x0 + 1)): one 16-byte load
+// then serves both, and only the other cases issue a second load.  Same
+// interpolation arithmetic and order as grid_level.
+template <int F>
+__device__ __forceinline__ void grid_level_h16(const GridLevels& lv, uint32_t level, float x0, float x1, float x2,
+                                               const half_t* __restrict__ table, float* v) {
+    constexpr uint32_t PER_BLOCK = 8 / F;  // entries per 16-byte block
+    const float scale = lv.scale[level];
+    const uint32_t res = lv.res[level];
+    const uint32_t off = lv.offset[level];
+    const uint32_t hs = lv.offset[level + 1] - off;
+    const half_t* tab = table + (size_t)off * F;
+    float fr[3];
+    uint32_t cell[3];
+    grid_pos(x0, scale, lv.interp, fr[0], cell[0]);
+    grid_pos(x1, scale, lv.interp, fr[1], cell[1]);
+    grid_pos(x2, scale, lv.interp, fr[2], cell[2]);
+#pragma unroll
+    for (int f = 0; f < F; ++f) v[f] = 0.0f;
+    if (lv.interp == INTERP_NEAREST) {
+        const uint32_t idx = grid_index(lv.grid_type, hs, res, cell[0], cell[1], cell[2]);
+        load_entry<F, half_t>(tab + (size_t)idx * F, v);
+        return;
+    }
+    float vals[8][F];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {  // p = (dy, dz); corners c = 2p (x0) and 2p + 1 (x0 + 1)
+        const uint32_t y = cell[1] + (p & 1), z = cell[2] + ((p >> 1) & 1);
+        const uint32_t i0 = grid_index(lv.grid_type, hs, res, cell[0], y, z);
+        const uint32_t i1 = grid_index(lv.grid_type, hs, res, cell[0] + 1, y, z);
+        const uint4 blk = *reinterpret_cast<const uint4*>(tab + (size_t)(i0 / PER_BLOCK) * PER_BLOCK * F);
+        entry_of_block<F>(blk, i0 % PER_BLOCK, vals[2 * p]);
+        if (i0 / PER_BLOCK == i1 / PER_BLOCK) entry_of_block<F>(blk, i1 % PER_BLOCK, vals[2 * p + 1]);
+        else load_entry<F, half_t>(tab + (size_t)i1 * F, vals[2 * p + 1]);
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) {

@@ -7,9 +7,11 @@
 //     XCD's 4 MiB L2 while 2^20 queries stream through it (the reference's
 //     "one level per block" idea, grid.h:1053-1058).  The last blockIdx.y row
 //     writes the non-grid features (spherical harmonics, identity, padding).
-//     Features are stored FP16 in the MFMA-fragment layout
-//     [N/16][K/8][16 queries][8 features], so phase 2 reads each B fragment
-//     with one coalesced 16-byte load per lane.
+//     Features are stored FP16 as [N/16][K/2][16 queries][2 features]: one
+//     level of a 16-query tile is one contiguous 64-byte row, written whole
+//     (a layout with 8 features per query row was written a quarter line at a
+//     time by four different levels: 3.3x the useful bytes reached HBM).
+//     Phase 2 reads a B fragment as 4 dwords, each coalesced across 16 lanes.
 //  2. MLP.  Every wave takes tiles of 16 queries; all layers run on
 //     v_mfma_f32_16x16x32_f16 with FP32 accumulation, weights resident in
 //     LDS, activations never leaving registers: the 16x16 accumulator layout
@@ -44,8 +46,13 @@ struct LevelMap {
     uint8_t level[kMaxLevelRows];
 };
 
-__device__ __forceinline__ size_t feat_index(uint32_t q, uint32_t f, uint32_t kp8) {
-    return (((size_t)(q >> 4) * kp8 + (f >> 3)) * 16 + (q & 15)) * 8 + (f & 7);
+// pair of 2 halves holding feature f of query q
+__device__ __forceinline__ uint32_t* pair_ptr(half_t* feat, uint32_t q, uint32_t f, uint32_t kp2) {
+    return reinterpret_cast<uint32_t*>(feat + feat_pair_offset(q >> 4, f >> 1, kp2, q & 15));
+}
+
+__device__ __forceinline__ uint32_t pack2(half_t a, half_t b) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
 }
 
 template <int F, int MAXD, typename TT>
@@ -56,7 +63,7 @@ __global__ __launch_bounds__(kBlock) void encode_kernel(uint32_t N, uint32_t N16
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= N16) return;
     const EncProgram& P = *prog;
-    const uint32_t kp8 = P.k_padded / 8;
+    const uint32_t kp2 = P.k_padded / 2, kp8 = P.k_padded / 8;
     const bool valid = q < N;
     const float* x = in + (size_t)(valid ? q : 0) * P.n_input_dims;
     if (blockIdx.y < map.n) {
@@ -64,24 +71,19 @@ __global__ __launch_bounds__(kBlock) void encode_kernel(uint32_t N, uint32_t N16
         const uint32_t level = map.level[blockIdx.y];
         const float* xs = x + s.in_offset;
         float v[F];
-        grid_level<F, TT>(s.lv, level, xs[0], xs[1], xs[2], table + s.param_offset, v);
-        const uint32_t f0 = s.out_offset + level * F;  // multiple of F: the F halves stay in one group
+        if constexpr (std::is_same<TT, half_t>::value)
+            grid_level_h16<F>(s.lv, level, xs[0], xs[1], xs[2], table + s.param_offset, v);
+        else
+            grid_level<F, TT>(s.lv, level, xs[0], xs[1], xs[2], table + s.param_offset, v);
+        const uint32_t f0 = s.out_offset + level * F;  // multiple of F
         half_t h[F];
 #pragma unroll
         for (int f = 0; f < F; ++f) h[f] = valid ? (half_t)v[f] : (half_t)0.0f;
-        half_t* dst = feat + feat_index(q, f0, kp8);
         if constexpr (F == 1) {
-            dst[0] = h[0];
-        } else if constexpr (F == 2) {
-            typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-            *reinterpret_cast<h2*>(dst) = h2{h[0], h[1]};
-        } else if constexpr (F == 4) {
-            *reinterpret_cast<half4*>(dst) = half4{h[0], h[1], h[2], h[3]};
+            feat[feat_pair_offset(q >> 4, f0 >> 1, kp2, q & 15) + (f0 & 1)] = h[0];
         } else {
-            half8 o;
 #pragma unroll
-            for (int f = 0; f < 8; ++f) o[f] = h[f];
-            *reinterpret_cast<half8*>(dst) = o;
+            for (int j = 0; j < F / 2; ++j) *pair_ptr(feat, q, f0 + 2 * j, kp2) = pack2(h[2 * j], h[2 * j + 1]);
         }
         return;
     }
@@ -91,16 +93,19 @@ __global__ __launch_bounds__(kBlock) void encode_kernel(uint32_t N, uint32_t N16
         if (mask == 0xffu) continue;
         float win[8];
         encode_window_f32<F, MAXD, TT, true>(P, table, x, 8 * g8, win);
-        half_t* dst = feat + feat_index(q, 8 * g8, kp8);
-        if (mask == 0) {
-            half8 o;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) o[j] = valid ? (half_t)win[j] : (half_t)0.0f;
-            *reinterpret_cast<half8*>(dst) = o;
-        } else {
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (!((mask >> j) & 1u)) dst[j] = valid ? (half_t)win[j] : (half_t)0.0f;
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t m2 = (mask >> (2 * j)) & 3u;
+            const half_t a0 = valid ? (half_t)win[2 * j] : (half_t)0.0f;
+            const half_t a1 = valid ? (half_t)win[2 * j + 1] : (half_t)0.0f;
+            const uint32_t f = 8 * g8 + 2 * j;
+            if (m2 == 0) {
+                *pair_ptr(feat, q, f, kp2) = pack2(a0, a1);
+            } else if (m2 != 3u) {  // one row of the pair belongs to a grid level
+                half_t* d = feat + feat_pair_offset(q >> 4, f >> 1, kp2, q & 15);
+                if (!(m2 & 1u)) d[0] = a0;
+                if (!(m2 & 2u)) d[1] = a1;
+            }
         }
     }
 }
@@ -121,16 +126,15 @@ __global__ __launch_bounds__(kBlock) void mlp_fwd_kernel(MlpArgs16 a, const half
         reinterpret_cast<float4*>(sw)[k] = reinterpret_cast<const float4*>(packed)[k];
     __syncthreads();
     const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
-    const uint32_t kc0 = a.k_padded / 32, kp8 = a.k_padded / 8;
+    const uint32_t kc0 = a.k_padded / 32, kp2 = a.k_padded / 2;
     const uint32_t n_tiles = (a.N + 15) / 16;
     const uint32_t n_waves = gridDim.x * (kBlock / 64);
     for (uint32_t tile = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); tile < n_tiles; tile += n_waves) {
         floatx4 acc[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) acc[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-        const half_t* fb = feat + (size_t)tile * kp8 * 128;
         for (uint32_t c = 0; c < kc0; ++c) {
-            const half8 b = *reinterpret_cast<const half8*>(fb + ((size_t)(4 * c + g) * 16 + n) * 8);
+            const half8 b = load_feat_frag(feat, tile, kp2, c, g, n);
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 const half8 w = *reinterpret_cast<const half8*>(sw + ((size_t)(mt * kc0 + c) * 64 + lane) * 8);

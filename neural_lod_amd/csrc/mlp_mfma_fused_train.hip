@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
     __syncthreads();
 
     const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15, wave = threadIdx.x >> 6;
-    const uint32_t kc0 = a.k_padded / 32, kp8 = a.k_padded / 8;
+    const uint32_t kc0 = a.k_padded / 32, kp2 = a.k_padded / 2;
     const uint32_t KT = (a.k_enc + 15) / 16;  // layer-0 dW column tiles
     const uint32_t n_tiles = (a.N + 15) / 16;
     const uint32_t rounds = (n_tiles + gridDim.x * kWaves - 1) / (gridDim.x * kWaves);
@@ -142,11 +142,10 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
         floatx4 acc[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) acc[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-        const half_t* fb = feat + (size_t)(tile_ok ? tile : 0) * kp8 * 128;
 #pragma unroll
         for (uint32_t c = 0; c < 4; ++c) {
             if (c >= kc0) break;
-            xb[c] = tile_ok ? *reinterpret_cast<const half8*>(fb + ((size_t)(4 * c + g) * 16 + n) * 8) : half8{};
+            xb[c] = tile_ok ? load_feat_frag(feat, tile, kp2, c, g, n) : half8{};
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 const half8 w = *reinterpret_cast<const half8*>(wf + ((size_t)(mt * kc0 + c) * 64 + lane) * 8);

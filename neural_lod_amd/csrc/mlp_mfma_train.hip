@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kBlock) void mlp_fwd_train_kernel(TrainArgs a, cons
         reinterpret_cast<float4*>(sw)[k] = reinterpret_cast<const float4*>(packed)[k];
     __syncthreads();
     const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
-    const uint32_t kc0 = a.k_padded / 32, kp8 = a.k_padded / 8;
+    const uint32_t kc0 = a.k_padded / 32, kp2 = a.k_padded / 2;
     const uint32_t n_tiles = (a.N + 15) / 16;
     const uint32_t n_waves = gridDim.x * (kBlock / 64);
     const size_t layer_stride = (size_t)n_tiles * W * 16;
@@ -66,9 +66,8 @@ __global__ __launch_bounds__(kBlock) void mlp_fwd_train_kernel(TrainArgs a, cons
         floatx4 acc[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) acc[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-        const half_t* fb = feat + (size_t)tile * kp8 * 128;
         for (uint32_t c = 0; c < kc0; ++c) {
-            const half8 b = *reinterpret_cast<const half8*>(fb + ((size_t)(4 * c + g) * 16 + n) * 8);
+            const half8 b = load_feat_frag(feat, tile, kp2, c, g, n);
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 const half8 w = *reinterpret_cast<const half8*>(sw + ((size_t)(mt * kc0 + c) * 64 + lane) * 8);
@@ -268,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void wgrad_mfma_kernel(TrainArgs a, const h
     const uint32_t rows = layer == L ? 16u : (uint32_t)W;
     const uint32_t cols = layer == 0 ? a.k_padded : (uint32_t)W;
     const uint32_t rt = rows / 16, ct = cols / 16, n_out_tiles = rt * ct;
-    const uint32_t kp8 = a.k_padded / 8;
+    const uint32_t kp2 = a.k_padded / 2;
     const half_t* Dl = D + (size_t)layer * layer_stride;  // layer L: the output deltas
     const uint32_t d_units = layer == L ? 16u : (uint32_t)W;
     const half_t* Hp = layer == 0 ? nullptr : H + (size_t)(layer - 1) * layer_stride;
@@ -289,10 +288,10 @@ __global__ __launch_bounds__(kBlock) void wgrad_mfma_kernel(TrainArgs a, const h
                 const half4_t av = *reinterpret_cast<const half4_t*>(Dl + unit_major(qt, d_units, 16 * mr + m16, 4 * g));
                 half4_t bv;
                 const uint32_t k = 16 * nc + m16;
-                if (layer == 0) {  // B = X[k][q], X in fragment layout
-                    const half_t* fb = feat + (((size_t)qt * kp8 + (k >> 3)) * 16) * 8 + (k & 7);
+                if (layer == 0) {  // B = X[k][q], X in the pair layout
+                    const half_t* fb = feat + feat_pair_offset(qt, k >> 1, kp2, 0) + (k & 1);
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) bv[j] = fb[(4 * g + j) * 8];
+                    for (int j = 0; j < 4; ++j) bv[j] = fb[(4 * g + j) * 2];
                 } else {
                     bv = *reinterpret_cast<const half4_t*>(Hp + unit_major(qt, W, k, 4 * g));
                 }

@@ -83,7 +83,7 @@ def _features(m, L, xt):
     kp = (L.net.n_input + 31) // 32 * 32
     buf = m.encode_fragments(xt).view(torch.float16).cpu().numpy().astype(np.float64)
     tiles = (n + 15) // 16
-    frag = buf[:tiles * kp * 16].reshape(tiles, kp // 8, 16, 8)
+    frag = buf[:tiles * kp * 16].reshape(tiles, kp // 2, 16, 2)  # [tile][feature pair][query][2]
     return frag.transpose(0, 2, 1, 3).reshape(tiles * 16, kp)[:n, :L.net.n_input]
 
 

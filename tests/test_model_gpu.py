@@ -231,7 +231,7 @@ def test_torch_module_gradcheck(cuda):
     with torch.no_grad():
         mod.params.copy_(torch.from_numpy(M.xavier_uniform(L, 21, grid_range=0.5)))
     x = torch.from_numpy(_inputs(cfg, 256, 22)).cuda()
-    w = torch.randn(256, 1, device="cuda")
+    w = torch.from_numpy(np.random.default_rng(24).normal(size=(256, 1)).astype(np.float32)).cuda()
     loss = (mod(x) * w).sum()
     loss.backward()
     g = mod.params.grad.detach().cpu().numpy().astype(np.float64)
@@ -243,16 +243,26 @@ def test_torch_module_gradcheck(cuda):
     grid_g = np.abs(g_ref[L.n_net_params:])
     big_grid = L.n_net_params + np.argsort(-grid_g)[:500]  # touched table entries with large gradient
     idx = np.concatenate([rng.choice(L.n_net_params, 20, replace=False), rng.choice(big_grid, 20, replace=False)])
+
+    def fd(i, h):
+        p0 = mod.params[i].item()
+        mod.params[i] = p0 + h
+        lp = (mod(x) * w).sum().item()
+        mod.params[i] = p0 - h
+        lm = (mod(x) * w).sum().item()
+        mod.params[i] = p0
+        return (lp - lm) / (2 * h)
+
+    checked = 0
     with torch.no_grad():
         for i in idx:
-            p0 = mod.params[i].item()
-            mod.params[i] = p0 + 1e-3
-            lp = (mod(x) * w).sum().item()
-            mod.params[i] = p0 - 1e-3
-            lm = (mod(x) * w).sum().item()
-            mod.params[i] = p0
-            fd = (lp - lm) / 2e-3
-            assert abs(fd - g[i]) <= 2e-3 * max(1.0, abs(g[i])) + 5e-3 * abs(g[i]), (i, fd, g[i])
+            tol = 2e-3 * max(1.0, abs(g[i])) + 5e-3 * abs(g[i])
+            f1, f2 = fd(i, 1e-3), fd(i, 5e-4)
+            if abs(f1 - f2) > tol:
+                continue  # a ReLU kink inside [p - h, p + h]: central differences are not valid there
+            checked += 1
+            assert abs(f1 - g[i]) <= tol, (i, f1, f2, g[i])
+    assert checked >= 30, f"only {checked} of {len(idx)} parameters away from kinks"
 
 
 def test_invalid_calls(cuda):
