@@ -116,10 +116,12 @@ struct MlpArgs16 {
     uint32_t packed_halves;
 };
 
-template <int W>
+// TT tiles of 16 queries per wave iteration: each weight fragment read from
+// LDS feeds TT MFMAs (LDS bandwidth for the weights bounds the single-tile loop).
+template <int W, int TT>
 __global__ __launch_bounds__(kBlock) void mlp_fwd_kernel(MlpArgs16 a, const half_t* __restrict__ packed,
                                                          const half_t* __restrict__ feat, float* __restrict__ out) {
-    constexpr int MT = W / 16;
+    constexpr int MT = W / 16, KC = W / 32;
     extern __shared__ float4 smem4[];
     half_t* sw = reinterpret_cast<half_t*>(smem4);
     for (uint32_t k = threadIdx.x; k < a.packed_halves / 8; k += kBlock)
@@ -128,26 +130,74 @@ __global__ __launch_bounds__(kBlock) void mlp_fwd_kernel(MlpArgs16 a, const half
     const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
     const uint32_t kc0 = a.k_padded / 32, kp2 = a.k_padded / 2;
     const uint32_t n_tiles = (a.N + 15) / 16;
+    const uint32_t n_groups = (n_tiles + TT - 1) / TT;
     const uint32_t n_waves = gridDim.x * (kBlock / 64);
-    for (uint32_t tile = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); tile < n_tiles; tile += n_waves) {
-        floatx4 acc[MT];
+    for (uint32_t grp = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); grp < n_groups; grp += n_waves) {
+        floatx4 acc[TT][MT];
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) acc[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < TT; ++t)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) acc[t][mt] = floatx4{0.f, 0.f, 0.f, 0.f};
         for (uint32_t c = 0; c < kc0; ++c) {
-            const half8 b = load_feat_frag(feat, tile, kp2, c, g, n);
+            half8 b[TT];
+#pragma unroll
+            for (int t = 0; t < TT; ++t) {
+                const uint32_t tile = min(grp * TT + t, n_tiles - 1);  // a clamped duplicate is not stored
+                b[t] = load_feat_frag(feat, tile, kp2, c, g, n);
+            }
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 const half8 w = *reinterpret_cast<const half8*>(sw + ((size_t)(mt * kc0 + c) * 64 + lane) * 8);
-                acc[mt] = mfma16(w, b, acc[mt]);
+#pragma unroll
+                for (int t = 0; t < TT; ++t) acc[t][mt] = mfma16(w, b[t], acc[t][mt]);
             }
         }
-        const floatx4 o = mlp_tail16<W>(sw + (size_t)MT * kc0 * 512, acc, a.n_hidden_layers, a.act);
-        const uint32_t q = tile * 16 + n;
-        if (q < a.N) {
+        const half_t* frag = sw + (size_t)MT * kc0 * 512;
+        half8 hb[TT][KC];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const uint32_t oi = 4 * g + i;
-                if (oi < a.n_output) out[(size_t)q * a.n_output + oi] = act_fwd(a.out_act, o[i]);
+        for (int t = 0; t < TT; ++t)
+#pragma unroll
+            for (int c = 0; c < KC; ++c) hb[t][c] = pack_act(acc[t][2 * c], acc[t][2 * c + 1], a.act);
+        for (uint32_t layer = 1; layer < a.n_hidden_layers; ++layer) {
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                floatx4 s[TT];
+#pragma unroll
+                for (int t = 0; t < TT; ++t) s[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int c = 0; c < KC; ++c) {
+                    const half8 w = *reinterpret_cast<const half8*>(frag + ((size_t)(mt * KC + c) * 64 + lane) * 8);
+#pragma unroll
+                    for (int t = 0; t < TT; ++t) s[t] = mfma16(w, hb[t][c], s[t]);
+                }
+#pragma unroll
+                for (int t = 0; t < TT; ++t) acc[t][mt] = s[t];
+            }
+            frag += (size_t)MT * KC * 512;
+#pragma unroll
+            for (int t = 0; t < TT; ++t)
+#pragma unroll
+                for (int c = 0; c < KC; ++c) hb[t][c] = pack_act(acc[t][2 * c], acc[t][2 * c + 1], a.act);
+        }
+        floatx4 o[TT];
+#pragma unroll
+        for (int t = 0; t < TT; ++t) o[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            const half8 w = *reinterpret_cast<const half8*>(frag + ((size_t)c * 64 + lane) * 8);
+#pragma unroll
+            for (int t = 0; t < TT; ++t) o[t] = mfma16(w, hb[t][c], o[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < TT; ++t) {
+            const uint32_t tile = grp * TT + t;
+            const uint32_t q = tile * 16 + n;
+            if (tile < n_tiles && q < a.N) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t oi = 4 * g + i;
+                    if (oi < a.n_output) out[(size_t)q * a.n_output + oi] = act_fwd(a.out_act, o[t][i]);
+                }
             }
         }
     }
@@ -232,16 +282,17 @@ void mfma_mlp_forward(hipStream_t stream, const Layout& L, const void* packed, c
     a.packed_halves = (uint32_t)packed_halves(L.net);
     const size_t smem = mfma_fused_smem_bytes(L.net);
     const uint32_t n_tiles = div_round_up(N, 16);
-    auto launch = [&](auto kern) {
+    auto launch = [&](auto kern, uint32_t tt) {
         const int occ = occupancy_blocks((const void*)kern, kBlock, smem);
-        const uint32_t blocks = std::min<uint32_t>(div_round_up(n_tiles, kBlock / 64), (uint32_t)(cu_count() * occ));
+        const uint32_t groups = div_round_up(n_tiles, tt);
+        const uint32_t blocks = std::min<uint32_t>(div_round_up(groups, kBlock / 64), (uint32_t)(cu_count() * occ));
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), smem, stream, a, static_cast<const half_t*>(packed),
                            static_cast<const half_t*>(feat), out);
     };
     switch (L.net.width) {
-        case 32: launch(mlp_fwd_kernel<32>); break;
-        case 64: launch(mlp_fwd_kernel<64>); break;
-        case 128: launch(mlp_fwd_kernel<128>); break;
+        case 32: launch(mlp_fwd_kernel<32, 2>, 2); break;
+        case 64: launch(mlp_fwd_kernel<64, 2>, 2); break;
+        case 128: launch(mlp_fwd_kernel<128, 1>, 1); break;
         default: fail(NLOD_ERR_UNSUPPORTED, "FP16 MFMA kernels support widths 32, 64 and 128");
     }
     NLOD_LAUNCH_CHECK();
