@@ -269,6 +269,46 @@ int nlod_adam_step(void* stream, const nlod_adam_config* config, uint64_t n, uin
                    uint32_t optimizer_step, float* d_weights, float* d_gradients, float* d_first_moments,
                    float* d_second_moments, uint32_t* d_param_steps);
 
+/* ------------------------------------------------------- wavefront queries */
+
+typedef struct nlod_wavefront nlod_wavefront;
+
+/* Engine for the per-wavefront queries of the neural LoD path tracer
+ * (rptr/cuda/neural-lod/wavefront_neural_throughput_visibility_lod.cu:464-720):
+ * per ray segment the visibility network, then, where the visibility reaches
+ * `threshold` (check_visibility, :597-610), the throughput network (:656-662).
+ * Both models must share a precision; their parameters must be set.  FP16
+ * models of width 32/64 with one feature count run one persistent fused
+ * launch per wavefront; otherwise the plain launches per network. */
+int nlod_wavefront_create(nlod_model* visibility, nlod_model* throughput, float threshold, nlod_wavefront** out);
+int nlod_wavefront_destroy(nlod_wavefront* wavefront);
+
+/* Autotuner: times the launch shapes (split launches; fused kernel with 1..max
+ * blocks per CU) once per wavefront-size bucket up to max_n queries and keeps
+ * the fastest.  Synchronizes the stream.  Untuned engines use the fused kernel
+ * at 1 block per CU where it applies. */
+int nlod_wavefront_tune(nlod_wavefront* wavefront, void* stream, uint32_t max_n);
+/* The shape the engine picks for n queries (0 = split launches, k = fused kernel, k blocks per CU). */
+int nlod_wavefront_choice(const nlod_wavefront* wavefront, uint32_t n, int* shape_out);
+int nlod_wavefront_max_shape(const nlod_wavefront* wavefront, int* max_shape_out);
+
+/* One wavefront of n segments: d_vis_in float[n][vis inputs], d_thr_in
+ * float[n][thr inputs] -> d_vis_out float[n][vis outputs], d_thr_out
+ * float[n][thr outputs] (0 where the visibility is below the threshold).
+ * Asynchronous.  _shape forces a launch shape (0 split, k > 0 fused). */
+int nlod_wavefront_run(nlod_wavefront* wavefront, void* stream, uint32_t n, const float* d_vis_in,
+                       const float* d_thr_in, float* d_vis_out, float* d_thr_out);
+int nlod_wavefront_run_shape(nlod_wavefront* wavefront, void* stream, uint32_t n, const float* d_vis_in,
+                             const float* d_thr_in, float* d_vis_out, float* d_thr_out, int shape);
+
+/* A sequence of n_batches wavefronts (counts on the host) stored back to back
+ * in the buffers.  use_graph: the sequence is captured in a hipGraph on the
+ * first call with these sizes and buffers (that call runs eagerly) and
+ * replayed by later calls. */
+int nlod_wavefront_run_batches(nlod_wavefront* wavefront, void* stream, uint32_t n_batches, const uint32_t* counts,
+                               const float* d_vis_in, const float* d_thr_in, float* d_vis_out, float* d_thr_out,
+                               int use_graph);
+
 #ifdef __cplusplus
 }
 #endif

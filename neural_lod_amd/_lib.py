@@ -88,6 +88,14 @@ SIGNATURES: dict[str, list] = {
     "nlod_trainer_deserialize": [_vp, _vp, C.c_size_t],
     "nlod_loss_evaluate": [_vp, _i32, _u64, C.c_float, _vp, _vp, _vp, _vp],
     "nlod_adam_step": [_vp, _vp, _u64, _u64, C.c_float, _u32, _vp, _vp, _vp, _vp, _vp],
+    "nlod_wavefront_create": [_vp, _vp, C.c_float, C.POINTER(_vp)],
+    "nlod_wavefront_destroy": [_vp],
+    "nlod_wavefront_tune": [_vp, _vp, _u32],
+    "nlod_wavefront_choice": [_vp, _u32, C.POINTER(_i32)],
+    "nlod_wavefront_max_shape": [_vp, C.POINTER(_i32)],
+    "nlod_wavefront_run": [_vp, _vp, _u32, _vp, _vp, _vp, _vp],
+    "nlod_wavefront_run_shape": [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _i32],
+    "nlod_wavefront_run_batches": [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _i32],
 }
 
 

@@ -26,6 +26,7 @@ struct GridArgs {
 // pos_fract (tiny-cuda-nn common_device.h:438-450): pos = x*scale + 0.5 as one
 // fused multiply-add, cell = floor(pos), fraction in float; smoothstep optional.
 __device__ __forceinline__ void grid_pos(float x, float scale, int interp, float& frac, uint32_t& cell) {
+#pragma clang fp contract(off)
     const float p = __builtin_fmaf(x, scale, 0.5f);
     const float fl = floorf(p);
     cell = (uint32_t)(int)fl;

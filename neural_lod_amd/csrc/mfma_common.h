@@ -115,6 +115,8 @@ __device__ __forceinline__ void load_entry(const TT* __restrict__ p, float* out)
 template <int F, typename TT = float>
 __device__ __forceinline__ void grid_level(const GridLevels& lv, uint32_t level, float x0, float x1, float x2,
                                            const TT* __restrict__ table, float* v) {
+    // no FMA contraction: the encoding rounds identically in every kernel it is inlined into
+#pragma clang fp contract(off)
     const float scale = lv.scale[level];
     const uint32_t res = lv.res[level];
     const uint32_t off = lv.offset[level];
@@ -190,6 +192,8 @@ __device__ __forceinline__ void entry_of_block(const uint4& blk, uint32_t e, flo
 template <int F>
 __device__ __forceinline__ void grid_level_h16(const GridLevels& lv, uint32_t level, float x0, float x1, float x2,
                                                const half_t* __restrict__ table, float* v) {
+    // no FMA contraction: the encoding rounds identically in every kernel it is inlined into
+#pragma clang fp contract(off)
     constexpr uint32_t PER_BLOCK = 8 / F;  // entries per 16-byte block
     const float scale = lv.scale[level];
     const uint32_t res = lv.res[level];
@@ -240,6 +244,8 @@ __device__ __forceinline__ void place(float* win, int j, float v) {
 template <int F, int MAXD, typename TT = float, bool SKIP_GRIDS = false>
 __device__ __forceinline__ void encode_window_f32(const EncProgram& P, const TT* __restrict__ params,
                                                   const float* __restrict__ q, uint32_t w0, float* win) {
+    // no FMA contraction: the encoding rounds identically in every kernel it is inlined into
+#pragma clang fp contract(off)
 #pragma unroll
     for (int j = 0; j < 8; ++j) win[j] = 0.0f;
     const uint32_t w1 = w0 + 8;
@@ -261,7 +267,10 @@ __device__ __forceinline__ void encode_window_f32(const EncProgram& P, const TT*
                 const int row = (int)r0 + l * F - (int)w0;
                 // gather unconditionally (clamped level: indices stay inside it), place if valid
                 float v[F];
-                grid_level<F, TT>(s.lv, (uint32_t)min(l, n_levels - 1), x0, x1, x2, table, v);
+                if constexpr (std::is_same<TT, half_t>::value)
+                    grid_level_h16<F>(s.lv, (uint32_t)min(l, n_levels - 1), x0, x1, x2, table, v);
+                else
+                    grid_level<F, TT>(s.lv, (uint32_t)min(l, n_levels - 1), x0, x1, x2, table, v);
                 const bool ok = l < n_levels && row >= 0 && row < 8;
 #pragma unroll
                 for (int f = 0; f < F; ++f) place(win, ok ? row + f : -1, v[f]);

@@ -332,3 +332,67 @@ class Module(torch.nn.Module):
 
     def forward(self, x):
         return _ModuleFunction.apply(self.native, x.contiguous(), self.params, self.loss_scale)
+
+
+class Wavefront:
+    """Per-wavefront visibility + throughput queries (libnlod's wavefront
+    engine): the visibility network, then the throughput network where the
+    visibility reaches ``threshold``."""
+
+    def __init__(self, visibility: NetworkWithInputEncoding, throughput: NetworkWithInputEncoding,
+                 threshold: float = 0.5):
+        h = C.c_void_p()
+        _lib.call("nlod_wavefront_create", visibility._h, throughput._h, threshold, C.byref(h))
+        self._h = h
+        self.visibility = visibility
+        self.throughput = throughput
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) is not None and self._h.value:
+                _lib.lib().nlod_wavefront_destroy(self._h)
+        except Exception:  # interpreter shutdown
+            pass
+        self._h = None
+
+    def tune(self, max_n: int, device=None) -> None:
+        _lib.call("nlod_wavefront_tune", self._h, _stream(device), max_n)
+
+    def choice(self, n: int) -> int:
+        c = C.c_int()
+        _lib.call("nlod_wavefront_choice", self._h, n, C.byref(c))
+        return c.value
+
+    def max_shape(self) -> int:
+        c = C.c_int()
+        _lib.call("nlod_wavefront_max_shape", self._h, C.byref(c))
+        return c.value
+
+    def _outputs(self, n, device):
+        return (torch.empty((n, self.visibility.n_output_dims), dtype=torch.float32, device=device),
+                torch.empty((n, self.throughput.n_output_dims), dtype=torch.float32, device=device))
+
+    def run(self, vis_in: torch.Tensor, thr_in: torch.Tensor, shape: int | None = None, out=None):
+        n = vis_in.shape[0]
+        _check_f32(vis_in, "vis_in", (n, self.visibility.n_input_dims))
+        _check_f32(thr_in, "thr_in", (n, self.throughput.n_input_dims))
+        vo, to = out if out is not None else self._outputs(n, vis_in.device)
+        if shape is None:
+            _lib.call("nlod_wavefront_run", self._h, _stream(vis_in.device), n, _ptr(vis_in), _ptr(thr_in), _ptr(vo),
+                      _ptr(to))
+        else:
+            _lib.call("nlod_wavefront_run_shape", self._h, _stream(vis_in.device), n, _ptr(vis_in), _ptr(thr_in),
+                      _ptr(vo), _ptr(to), shape)
+        return vo, to
+
+    def run_batches(self, counts, vis_in: torch.Tensor, thr_in: torch.Tensor, out=None, graph: bool = False):
+        n = vis_in.shape[0]
+        if sum(counts) != n:
+            raise ValueError("counts must sum to the number of rows")
+        _check_f32(vis_in, "vis_in", (n, self.visibility.n_input_dims))
+        _check_f32(thr_in, "thr_in", (n, self.throughput.n_input_dims))
+        vo, to = out if out is not None else self._outputs(n, vis_in.device)
+        c = (C.c_uint32 * len(counts))(*counts)
+        _lib.call("nlod_wavefront_run_batches", self._h, _stream(vis_in.device), len(counts), c, _ptr(vis_in),
+                  _ptr(thr_in), _ptr(vo), _ptr(to), int(graph))
+        return vo, to
