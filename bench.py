@@ -41,6 +41,86 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+WAVEFRONT_LODS = 5
+WAVEFRONT_BOUNCES = 4
+
+
+def wavefront_bench(dev, args):
+    """Uneven wavefronts: 5 LoD levels x 4 bounces, active rays per wavefront
+    log-uniform in [2^10, 2^20] (seeded); per segment the visibility network,
+    then the throughput network where visibility >= 0.5.  FP16: tuned fused
+    launches, the 20-wavefront sequence replayed from a hipGraph.  FP32 plain:
+    separate launches per network and wavefront, no graph."""
+    import torch
+    from neural_lod_amd import configs, synthetic
+    from neural_lod_amd.tcnn import FP16, FP32, NetworkWithInputEncoding, Wavefront
+    from oracle import model as M
+    rng = np.random.default_rng(4)
+    counts = [int(c) for c in np.exp2(rng.uniform(10, 20, WAVEFRONT_LODS * WAVEFRONT_BOUNCES))]
+    total = sum(counts)
+    vis_in = torch.from_numpy(synthetic.visibility_queries(total, 41)).to(dev)
+    thr_q = synthetic.throughput_queries(total, seed=42)
+    # LoD input of each wavefront: its level index / 4 (levels 0..4, 4 bounces each)
+    off = 0
+    for i, c in enumerate(counts):
+        thr_q[off:off + c, 9] = (i // WAVEFRONT_BOUNCES) / 4.0
+        off += c
+    thr_in = torch.from_numpy(thr_q).to(dev)
+    res = {"wavefronts": len(counts), "counts": counts, "segments": total, "threshold": 0.5}
+    for name, prec in (("fp32", FP32), ("fp16", FP16)):
+        nets = []
+        for cfg, seed in ((configs.VISIBILITY, 5), (configs.THROUGHPUT, 6)):
+            m = NetworkWithInputEncoding(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"], prec)
+            L = M.Layout(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"])
+            p = torch.from_numpy(M.xavier_uniform(L, seed, grid_range=1.0)).to(dev)
+            m.set_params(p)
+            nets.append((m, p))
+        wf = Wavefront(nets[0][0], nets[1][0], 0.5)
+        if prec == FP16:
+            wf.tune(max(counts))
+        vo = torch.empty((total, 1), device=dev)
+        to = torch.empty((total, 3), device=dev)
+        use_graph = prec == FP16
+        for _ in range(2):  # warm (and capture)
+            wf.run_batches(counts, vis_in, thr_in, out=(vo, to), graph=use_graph)
+        torch.cuda.synchronize()
+        reps = max(3, args.steps // 5)
+        stream = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            wf.run_batches(counts, vis_in, thr_in, out=(vo, to), graph=use_graph)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        seq_ms = e0.elapsed_time(e1) / reps
+        # per-wavefront latency: each wavefront alone, median of 5
+        lat = []
+        off = 0
+        for c in counts:
+            sl = slice(off, off + c)
+            v, t = vis_in[sl], thr_in[sl]
+            ms = []
+            for _ in range(5):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                wf.run(v, t, out=(vo[sl], to[sl]))
+                b.record(stream)
+                torch.cuda.synchronize()
+                ms.append(a.elapsed_time(b))
+            lat.append(float(np.median(ms)))
+            off += c
+        res[name] = {"queries_per_s": total / (seq_ms * 1e-3), "sequence_ms": seq_ms,
+                     "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
+                     "graph": use_graph}
+        if prec == FP16:
+            res[name]["shapes"] = [wf.choice(c) for c in counts]
+        log(f"wavefront {name}: {seq_ms:.3f} ms / sequence, {total / seq_ms / 1e6:.2f} Mq/ms..., "
+            f"p50 {res[name]['latency_ms_p50']:.3f} ms p99 {res[name]['latency_ms_p99']:.3f} ms")
+        del wf, nets
+    res["speedup_fp16_vs_fp32"] = res["fp32"]["sequence_ms"] / res["fp16"]["sequence_ms"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,6 +129,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-train", action="store_true", help="skip the training-step measurement")
     ap.add_argument("--train-steps", type=int, default=50)
+    ap.add_argument("--no-wavefront", action="store_true", help="skip the wavefront-batch measurement")
     ap.add_argument("--profile-encode", action="store_true",
                     help="only run the FP16 encoding pass and a 256 MiB device copy (for counter profiles)")
     args = ap.parse_args()
@@ -176,6 +257,8 @@ def main():
         train["batch"] = nb
         train["labels"] = "analytic sphere occlusion of random rays (synthetic)"
 
+    wavefront = None if args.no_wavefront else wavefront_bench(dev, args)
+
     # CPU baseline: the oracle's FP32 C loop on a bounded sample
     cpu = None
     if not args.no_cpu:
@@ -225,6 +308,7 @@ def main():
                      "mlp_tflops": FLOP_PER_QUERY * N_QUERIES / ((fp16["ms"] - enc_ms) * 1e-3) / 1e12},
         "cpu_baseline": cpu,
         "train": train,
+        "wavefront": wavefront,
     }
     print(json.dumps(line), flush=True)
 
