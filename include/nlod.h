@@ -60,7 +60,11 @@ enum nlod_vox_flags {
     NLOD_VOX_SOLID = 1u << 0,  /* parity solid fill (voxelize_solid.cu:73-146, -solid) */
     NLOD_VOX_MORTON = 1u << 1, /* Morton-ordered bits (-o morton); needs a power-of-two grid */
     NLOD_VOX_PAD_EPS = 1u << 2,/* pad the cube by 1e-4 of its extent (main.cpp:45,209-215, -pad) */
-    NLOD_VOX_ACCUMULATE = 1u << 3 /* do not clear the table first */
+    NLOD_VOX_ACCUMULATE = 1u << 3, /* do not clear the table first */
+    /* Kernel choice (same bits either way).  Default: LDS-tiled for row-major
+     * tables of >= 4096 triangles with at most 16384 tiles, else direct. */
+    NLOD_VOX_DIRECT = 1u << 4, /* one triangle per lane, global atomics */
+    NLOD_VOX_TILED = 1u << 5   /* LDS-tiled whenever the table allows it, at any triangle count */
 };
 
 /* Number of uint32 words of a res^3 bit table: ceil(res^3 / 32)

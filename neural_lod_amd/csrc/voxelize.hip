@@ -7,11 +7,32 @@
 // the same IEEE single (or, where the reference mixes in double literals,
 // double) operation in the same order, with contraction disabled.
 //
-// GPU design: a workgroup of 256 lanes streams 256 consecutive triangles
-// through LDS with coalesced loads (36 B per triangle, read as one contiguous
-// 9 KiB run), then each lane owns one triangle.  Set bits are merged per
-// 32-bit word in a register before one atomicOr per word, so a row of x
-// candidates that falls in one word costs one atomic instead of one per voxel.
+// Two kernel paths give the same bits:
+//
+// Direct: each lane owns one triangle.  Surface mode sweeps x innermost, so
+// the set bits of one row merge into one global atomic per 32-bit word.
+// Solid mode XORs each sample row [0, xmax] into the table with one atomic
+// per word; at about 21 G/s for random 32-bit global atomics on MI355X that
+// is its bound.
+//
+// Tiled (row-major tables): the grid is cut into tiles of T x T voxel rows
+// (full x extent) small enough for LDS, where the bits are set by LDS atomics:
+//   1. count    per (block of triangles, tile): list entries, from the same
+//               clamped row range the per-triangle routine walks; the tile
+//               range of each triangle is kept for step 3
+//   2. scan     tile-major exclusive scans -> list slots and chunk numbers
+//   3. scatter  triangle indices into their tiles' lists
+//   4. tiles    one workgroup per chunk of a tile's list runs the reference's
+//               routine restricted to the tile's rows into an LDS copy of the
+//               tile, then merges its nonzero words into the table (plain
+//               stores when the tile has one chunk and owns whole words,
+//               atomics otherwise).  Solid fill on word-aligned rows stores one
+//               toggle per sample at xmax and turns the toggles into the row
+//               ranges with a suffix-XOR scan per row (XOR is linear, so the
+//               chunks of a tile still combine by XOR).  Surface mode spreads
+//               the voxel columns of a batch of triangles over the lanes.
+// Triangles touching more than kMaxTiles tiles take the direct path after
+// the tiled pass; Morton-ordered output always does.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
@@ -19,6 +40,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -81,41 +104,97 @@ __device__ __forceinline__ uint64_t vox_index(const VoxParams& p, int x, int y, 
     return (uint64_t)x + (uint64_t)y * p.G + (uint64_t)z * p.G * p.G;
 }
 
-// Register-buffered bit writer: merges bits of the same word, one atomic per word.
+// Bit sinks: merge the bits of one 32-bit word in a register, then apply them
+// to the target with one OR (or XOR for the solid parity fill).
+__device__ __forceinline__ uint32_t range_mask(uint32_t lo, uint32_t hi) {
+    // bit b of a word (MSB-first) is index position 31-b: positions lo..hi
+    return (0xffffffffu >> lo) & (0xffffffffu << (31u - hi));
+}
+
+// Target: the whole table in global memory (atomics).
 template <bool XOR>
-struct WordWriter {
+struct GlobalSink {
     uint32_t* table;
     uint64_t word = ~0ull;
     uint32_t mask = 0;
+    __device__ void apply(uint64_t w, uint32_t m) {
+        if (XOR) atomicXor(table + w, m);
+        else atomicOr(table + w, m);
+    }
     __device__ void flush() {
-        if (mask) {
-            if (XOR) atomicXor(table + word, mask);
-            else atomicOr(table + word, mask);
-        }
+        if (mask) apply(word, mask);
         mask = 0;
     }
-    __device__ void set(uint64_t index) {
-        uint64_t w = index >> 5;
+    __device__ void set(uint64_t index, int) {
+        const uint64_t w = index >> 5;
         if (w != word) {
             flush();
             word = w;
         }
         mask |= 1u << (31u - (uint32_t)(index & 31u));
     }
-    // XOR (or OR) bits [first, last] of a contiguous index range.
-    __device__ void set_range(uint64_t first, uint64_t last) {
+    // bits of the contiguous index range [first, last]
+    __device__ void set_range(uint64_t first, uint64_t last, int) {
         flush();
-        uint64_t w0 = first >> 5, w1 = last >> 5;
-        for (uint64_t w = w0; w <= w1; ++w) {
-            uint32_t lo = (w == w0) ? (uint32_t)(first & 31u) : 0u;
-            uint32_t hi = (w == w1) ? (uint32_t)(last & 31u) : 31u;
-            // bit b of the word (MSB-first) is index position 31-b; positions lo..hi
-            uint32_t m = (0xffffffffu >> lo) & (0xffffffffu << (31u - hi));
-            if (XOR) atomicXor(table + w, m);
-            else atomicOr(table + w, m);
-        }
+        const uint64_t w0 = first >> 5, w1 = last >> 5;
+        for (uint64_t w = w0; w <= w1; ++w)
+            apply(w, range_mask(w == w0 ? (uint32_t)(first & 31u) : 0u, w == w1 ? (uint32_t)(last & 31u) : 31u));
         word = ~0ull;
     }
+};
+
+// Target: one tile's words in LDS.  Plane zz of the tile (z = z0 + zz) holds
+// the words covering voxel rows y0 .. y0 + T - 1 of that plane, starting at
+// word first_word(z); the LDS copy of a plane is `wpp` words long.
+// DIFF (solid fill of word-aligned rows): set_range(first, last) toggles only
+// the bit of `last`; a suffix-XOR scan along each row afterwards turns the
+// toggles into the same bits as XOR-ing every range [row start, last].
+template <bool XOR, bool DIFF = false>
+struct LdsSink {
+    uint32_t* lds;
+    uint64_t G;
+    int z0, y0;
+    uint32_t wpp;
+    uint64_t word = ~0ull;  // global word index
+    uint32_t mask = 0, slot = 0;
+    __device__ uint64_t first_word(int z) const { return ((uint64_t)z * G * G + (uint64_t)y0 * G) >> 5; }
+    __device__ void apply(uint32_t at, uint32_t m) {
+        if (XOR) atomicXor(lds + at, m);
+        else atomicOr(lds + at, m);
+    }
+    __device__ void flush() {
+        if (mask) apply(slot, mask);
+        mask = 0;
+    }
+    __device__ void set(uint64_t index, int z) {
+        const uint64_t w = index >> 5;
+        if (w != word) {
+            flush();
+            word = w;
+            slot = (uint32_t)(z - z0) * wpp + (uint32_t)(w - first_word(z));
+        }
+        mask |= 1u << (31u - (uint32_t)(index & 31u));
+    }
+    __device__ void set_range(uint64_t first, uint64_t last, int z) {
+        flush();
+        const uint32_t base = (uint32_t)(z - z0) * wpp;
+        const uint64_t fw = first_word(z);
+        if (DIFF) {
+            apply(base + (uint32_t)((last >> 5) - fw), 1u << (31u - (uint32_t)(last & 31u)));
+            word = ~0ull;
+            return;
+        }
+        const uint64_t w0 = first >> 5, w1 = last >> 5;
+        for (uint64_t w = w0; w <= w1; ++w)
+            apply(base + (uint32_t)(w - fw),
+                  range_mask(w == w0 ? (uint32_t)(first & 31u) : 0u, w == w1 ? (uint32_t)(last & 31u) : 31u));
+        word = ~0ull;
+    }
+};
+
+// Row range (inclusive) a per-triangle routine may touch; empty if y0 > y1.
+struct Rows {
+    int y0, y1, z0, z1;
 };
 
 __device__ __forceinline__ bool tri_finite(const float* t) {
@@ -125,9 +204,65 @@ __device__ __forceinline__ bool tri_finite(const float* t) {
     return ok;
 }
 
-// cpu_voxelizer.cpp:63-186 for one triangle (vertices t[0..8], untranslated).
-__device__ void surface_triangle(const VoxParams& p, const float* t, uint32_t* table) {
-    if (!tri_finite(t)) return;
+// Voxel range of a triangle in surface mode (cpu_voxelizer.cpp:70-84).
+__device__ Rows surface_rows(const VoxParams& p, const float* t) {
+    if (!tri_finite(t)) return {1, 0, 1, 0};
+    const f3 unit = {p.unit[0], p.unit[1], p.unit[2]};
+    const f3 bmin = {p.bmin[0], p.bmin[1], p.bmin[2]};
+    const float gm = (float)(p.G - 1);
+    f3 v0 = sub3({t[0], t[1], t[2]}, bmin);
+    f3 v1 = sub3({t[3], t[4], t[5]}, bmin);
+    f3 v2 = sub3({t[6], t[7], t[8]}, bmin);
+    const float ymn = gmin(v0.y, gmin(v1.y, v2.y)), zmn = gmin(v0.z, gmin(v1.z, v2.z));
+    const float ymx = gmax(v0.y, gmax(v1.y, v2.y)), zmx = gmax(v0.z, gmax(v1.z, v2.z));
+    return {(int)gclamp(ymn / unit.y, 0.0f, gm), (int)gclamp(ymx / unit.y, 0.0f, gm),
+            (int)gclamp(zmn / unit.z, 0.0f, gm), (int)gclamp(zmx / unit.z, 0.0f, gm)};
+}
+
+// Per-triangle constants of the surface test (cpu_voxelizer.cpp:63-136),
+// clipped bounding box and sweep set-up; kept in LDS so that the columns of
+// one triangle can run on different lanes.
+struct SurfTri {
+    float n[3];
+    float d1, d2;
+    f2 nv[3][3];     // edge normals of the projections xy, yz, zx
+    float dv[3][3];  // their offsets
+    float lo_ndp, hi_ndp, inv_k;
+    uint32_t range[3];  // per axis: lo | hi << 16 (clipped bounding box)
+    int axis;           // sweep axis (innermost), -1: no voxel to test
+    int slab;           // plane-test interval valid
+};
+
+__device__ __forceinline__ int range_lo(uint32_t r) { return (int)(r & 0xffffu); }
+__device__ __forceinline__ int range_hi(uint32_t r) { return (int)(r >> 16); }
+
+__device__ __forceinline__ bool edge_tests(const f2* nv, const float* dv, f2 q) {
+    // all three edge functions of one projection must be >= 0 (cpu_voxelizer.cpp:164-180)
+    return !((dot2(nv[0], q) + dv[0]) < 0.0f) && !((dot2(nv[1], q) + dv[1]) < 0.0f) &&
+           !((dot2(nv[2], q) + dv[2]) < 0.0f);
+}
+
+// cpu_voxelizer.cpp:63-136 for one triangle (vertices t[0..8], untranslated):
+// everything the voxel loop needs, with the loop restricted to rows y in
+// [clip.y0, clip.y1] and planes z in [clip.z0, clip.z1].
+//
+// Sweep: voxels whose plane test provably fails are skipped; every other
+// voxel runs the reference's tests unchanged, so the set bits are the same.
+// A voxel fails when (ndp + d1) and (ndp + d2) have one strict sign, which
+// holds when the exact ndp lies outside [L - M, H + M] (L, H = min, max of
+// -d1, -d2).  S bounds |n.x px| + |n.y py| + |n.z pz| over the table and M =
+// 1e-5 (S + |d1| + |d2|): the float rounding of ndp is below 4 * 2^-24 * S,
+// the float arithmetic that finds the interval (below) errs by less than
+// 1e-6 M, and M^2 is a normal float, so the rounded product is then > 0.
+// Sweeping along the axis where |n| is largest leaves a short interval per
+// column of the other two axes.
+//
+// x_only: always sweep along x (the reference's loop order; consecutive set
+// bits then fall in one word), with the interval only where |n.x| is largest.
+__device__ SurfTri surface_setup(const VoxParams& p, const float* t, const Rows clip, bool x_only = false) {
+    SurfTri st;
+    st.axis = -1;
+    if (!tri_finite(t)) return st;
     const f3 unit = {p.unit[0], p.unit[1], p.unit[2]};
     const f3 delta_p = unit;
     const f3 grid_max = {(float)(p.G - 1), (float)(p.G - 1), (float)(p.G - 1)};
@@ -141,11 +276,12 @@ __device__ void surface_triangle(const VoxParams& p, const float* t, uint32_t* t
     f3 wmin = {gmin(v0.x, gmin(v1.x, v2.x)), gmin(v0.y, gmin(v1.y, v2.y)), gmin(v0.z, gmin(v1.z, v2.z))};
     f3 wmax = {gmax(v0.x, gmax(v1.x, v2.x)), gmax(v0.y, gmax(v1.y, v2.y)), gmax(v0.z, gmax(v1.z, v2.z))};
     const int x0 = (int)gclamp(wmin.x / unit.x, 0.0f, grid_max.x);
-    const int y0 = (int)gclamp(wmin.y / unit.y, 0.0f, grid_max.y);
-    const int z0 = (int)gclamp(wmin.z / unit.z, 0.0f, grid_max.z);
+    const int y0 = max((int)gclamp(wmin.y / unit.y, 0.0f, grid_max.y), clip.y0);
+    const int z0 = max((int)gclamp(wmin.z / unit.z, 0.0f, grid_max.z), clip.z0);
     const int x1 = (int)gclamp(wmax.x / unit.x, 0.0f, grid_max.x);
-    const int y1 = (int)gclamp(wmax.y / unit.y, 0.0f, grid_max.y);
-    const int z1 = (int)gclamp(wmax.z / unit.z, 0.0f, grid_max.z);
+    const int y1 = min((int)gclamp(wmax.y / unit.y, 0.0f, grid_max.y), clip.y1);
+    const int z1 = min((int)gclamp(wmax.z / unit.z, 0.0f, grid_max.z), clip.z1);
+    if (y0 > y1 || z0 > z1) return st;
 
     f3 c = {0.0f, 0.0f, 0.0f};
     if (n.x > 0.0f) c.x = unit.x;
@@ -161,38 +297,155 @@ __device__ void surface_triangle(const VoxParams& p, const float* t, uint32_t* t
     f2 nzx[3] = {{-1.0f * e0.x, e0.z}, {-1.0f * e1.x, e1.z}, {-1.0f * e2.x, e2.z}};
     if (n.y < 0.0f) for (int k = 0; k < 3; ++k) nzx[k] = {-nzx[k].x, -nzx[k].y};
     const f3 vv[3] = {v0, v1, v2};
-    float dxy[3], dyz[3], dzx[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        dxy[k] = ((-1.0f * dot2(nxy[k], {vv[k].x, vv[k].y})) + gmax(0.0f, unit.x * nxy[k].x)) + gmax(0.0f, unit.y * nxy[k].y);
-        dyz[k] = ((-1.0f * dot2(nyz[k], {vv[k].y, vv[k].z})) + gmax(0.0f, unit.y * nyz[k].x)) + gmax(0.0f, unit.z * nyz[k].y);
+        st.nv[0][k] = nxy[k];
+        st.nv[1][k] = nyz[k];
+        st.nv[2][k] = nzx[k];
+        st.dv[0][k] = ((-1.0f * dot2(nxy[k], {vv[k].x, vv[k].y})) + gmax(0.0f, unit.x * nxy[k].x)) + gmax(0.0f, unit.y * nxy[k].y);
+        st.dv[1][k] = ((-1.0f * dot2(nyz[k], {vv[k].y, vv[k].z})) + gmax(0.0f, unit.y * nyz[k].x)) + gmax(0.0f, unit.z * nyz[k].y);
         // The CPU path weights the z component by unit.x and the x component
         // by unit.z here (cpu_voxelizer.cpp:133-135); bit-exactness keeps it.
-        dzx[k] = ((-1.0f * dot2(nzx[k], {vv[k].z, vv[k].x})) + gmax(0.0f, unit.x * nzx[k].x)) + gmax(0.0f, unit.z * nzx[k].y);
+        st.dv[2][k] = ((-1.0f * dot2(nzx[k], {vv[k].z, vv[k].x})) + gmax(0.0f, unit.x * nzx[k].x)) + gmax(0.0f, unit.z * nzx[k].y);
     }
+    st.n[0] = n.x;
+    st.n[1] = n.y;
+    st.n[2] = n.z;
+    st.d1 = d1;
+    st.d2 = d2;
+    st.range[0] = (uint32_t)x0 | ((uint32_t)x1 << 16);
+    st.range[1] = (uint32_t)y0 | ((uint32_t)y1 << 16);
+    st.range[2] = (uint32_t)z0 | ((uint32_t)z1 << 16);
 
-    WordWriter<false> out{table};
-    for (int z = z0; z <= z1; ++z) {
-        for (int y = y0; y <= y1; ++y) {
-            for (int x = x0; x <= x1; ++x) {
-                const f3 pt = {(float)x * unit.x, (float)y * unit.y, (float)z * unit.z};
-                const float ndp = dot3(n, pt);
-                if (((ndp + d1) * (ndp + d2)) > 0.0f) continue;
-                const f2 pxy = {pt.x, pt.y}, pyz = {pt.y, pt.z}, pzx = {pt.z, pt.x};
-                if ((dot2(nxy[0], pxy) + dxy[0]) < 0.0f) continue;
-                if ((dot2(nxy[1], pxy) + dxy[1]) < 0.0f) continue;
-                if ((dot2(nxy[2], pxy) + dxy[2]) < 0.0f) continue;
-                if ((dot2(nyz[0], pyz) + dyz[0]) < 0.0f) continue;
-                if ((dot2(nyz[1], pyz) + dyz[1]) < 0.0f) continue;
-                if ((dot2(nyz[2], pyz) + dyz[2]) < 0.0f) continue;
-                if ((dot2(nzx[0], pzx) + dzx[0]) < 0.0f) continue;
-                if ((dot2(nzx[1], pzx) + dzx[1]) < 0.0f) continue;
-                if ((dot2(nzx[2], pzx) + dzx[2]) < 0.0f) continue;
-                out.set(vox_index(p, x, y, z));
-            }
-        }
+    const float S = fabsf(n.x) * ((float)p.G * unit.x) + fabsf(n.y) * ((float)p.G * unit.y) +
+                    fabsf(n.z) * ((float)p.G * unit.z);
+    const float M = 1e-5f * ((S + fabsf(d1)) + fabsf(d2));
+    st.slab = isfinite(M) && S > 1e-12f;
+    st.lo_ndp = gmin(-d1, -d2) - M;
+    st.hi_ndp = gmax(-d1, -d2) + M;
+    int axis = 0;
+    if (st.slab) {
+        const float ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
+        axis = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
     }
-    out.flush();
+    if (x_only && axis != 0) {
+        axis = 0;
+        st.slab = 0;
+    }
+    st.axis = axis;
+    const float un[3] = {unit.x, unit.y, unit.z};
+    st.inv_k = 1.0f / (st.n[axis] * un[axis]);  // |n[axis]| >= 1/sqrt(3) when slab
+    return st;
+}
+
+// Number of columns (voxel lines along the sweep axis) of a set-up triangle.
+template <int A>
+__device__ __forceinline__ uint32_t surface_columns(const SurfTri& st) {
+    constexpr int U = A == 2 ? 1 : 2;
+    constexpr int V = A == 0 ? 1 : 0;
+    if (st.axis != A) return 0;
+    return (uint32_t)(range_hi(st.range[U]) - range_lo(st.range[U]) + 1) *
+           (uint32_t)(range_hi(st.range[V]) - range_lo(st.range[V]) + 1);
+}
+
+// Column j of a set-up triangle with sweep axis A: the projection tests that
+// do not involve A run once, the plane test bounds A to an interval, and each
+// voxel left runs the plane test and the remaining projection tests as the
+// reference loop body (cpu_voxelizer.cpp:148-183).
+template <int A, class Sink>
+__device__ void surface_column(const VoxParams& p, const SurfTri& st, uint32_t j, Sink& out) {
+    constexpr int U = A == 2 ? 1 : 2;  // outer axis
+    constexpr int V = A == 0 ? 1 : 0;  // inner column axis
+    const float un[3] = {p.unit[0], p.unit[1], p.unit[2]};
+    const uint32_t nv_len = (uint32_t)(range_hi(st.range[V]) - range_lo(st.range[V]) + 1);
+    const int u = range_lo(st.range[U]) + (int)(j / nv_len);
+    const int v = range_lo(st.range[V]) + (int)(j % nv_len);
+    const float pu = (float)u * un[U];
+    const float pv = (float)v * un[V];
+    // the projection onto the plane of U and V: (y, z), (z, x), (x, y)
+    if (A == 0 && !edge_tests(st.nv[1], st.dv[1], f2{pv, pu})) return;
+    if (A == 1 && !edge_tests(st.nv[2], st.dv[2], f2{pu, pv})) return;
+    if (A == 2 && !edge_tests(st.nv[0], st.dv[0], f2{pv, pu})) return;
+    int as = range_lo(st.range[A]), ae = range_hi(st.range[A]);
+    if (st.slab) {
+        const float C = st.n[U] * pu + st.n[V] * pv;
+        const float lo_k = st.lo_ndp - C, hi_k = st.hi_ndp - C;  // need n[A] un[A] a in [lo_k, hi_k]
+        float xa = lo_k * st.inv_k, xb = hi_k * st.inv_k;
+        if (st.inv_k < 0.0f) { const float tmp = xa; xa = xb; xb = tmp; }
+        if (xa > (float)ae || xb < (float)as) return;
+        if (xa > (float)as) as = max(as, (int)floorf(xa));
+        if (xb < (float)ae) ae = min(ae, (int)ceilf(xb));
+    }
+    const f3 n = {st.n[0], st.n[1], st.n[2]};
+    for (int a = as; a <= ae; ++a) {
+        int c[3];
+        c[A] = a;
+        c[U] = u;
+        c[V] = v;
+        const f3 pt = {(float)c[0] * un[0], (float)c[1] * un[1], (float)c[2] * un[2]};
+        const float ndp = dot3(n, pt);
+        if (((ndp + st.d1) * (ndp + st.d2)) > 0.0f) continue;
+        if (A != 2 && !edge_tests(st.nv[0], st.dv[0], f2{pt.x, pt.y})) continue;
+        if (A != 0 && !edge_tests(st.nv[1], st.dv[1], f2{pt.y, pt.z})) continue;
+        if (A != 1 && !edge_tests(st.nv[2], st.dv[2], f2{pt.z, pt.x})) continue;
+        out.set(vox_index(p, c[0], c[1], c[2]), c[2]);
+    }
+}
+
+// Exclusive prefix sum over the block (kVoxThreads lanes); returns the total.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_part, uint32_t& excl) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_part[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kVoxThreads / 64; ++w) {
+        const uint32_t pw = s_part[w];
+        if (w < wave) before += pw;
+        total += pw;
+    }
+    excl = before + x - v;
+    __syncthreads();  // s_part is reused by the next scan
+    return total;
+}
+
+// Surface voxelization of up to kVoxThreads set-up triangles held in s_st
+// (lane i set up triangle i): the columns of all of them are spread over the
+// block, one sweep axis at a time.  s_pre: kVoxThreads + 1 words.
+template <class Sink>
+__device__ void surface_block(const VoxParams& p, const SurfTri* s_st, uint32_t* s_pre, uint32_t* s_part,
+                              Sink& out) {
+    const SurfTri& mine = s_st[threadIdx.x];
+    auto phase = [&](auto axis_tag) {
+        constexpr int A = decltype(axis_tag)::value;
+        const uint32_t cols = surface_columns<A>(mine);
+        uint32_t excl;
+        const uint32_t total = block_exclusive_scan(cols, s_part, excl);
+        if (total == 0) return;
+        s_pre[threadIdx.x] = excl;
+        if (threadIdx.x == 0) s_pre[kVoxThreads] = total;
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < total; k += kVoxThreads) {
+            uint32_t lo = 0, hi = kVoxThreads;  // s_pre[lo] <= k < s_pre[hi]
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_pre[mid] <= k) lo = mid;
+                else hi = mid;
+            }
+            surface_column<A>(p, s_st[lo], k - s_pre[lo], out);
+        }
+        out.flush();
+        __syncthreads();  // s_pre is reused by the next phase
+    };
+    phase(std::integral_constant<int, 0>{});
+    phase(std::integral_constant<int, 1>{});
+    phase(std::integral_constant<int, 2>{});
 }
 
 // check_point_triangle (cpu_voxelizer.cpp:228-249)
@@ -214,17 +467,20 @@ __device__ __forceinline__ bool top_left_edge(f2 a, f2 b) {
     return (b.y < a.y) || (b.y == a.y && a.x > b.x);
 }
 
-// cpu_voxelizer.cpp:265-325 for one triangle.
-__device__ void solid_triangle(const VoxParams& p, const float* t, uint32_t* table) {
-    if (!tri_finite(t)) return;
+// cpu_voxelizer.cpp:265-325 for one triangle, restricted to sample rows y in
+// [clip.y0, clip.y1] and z in [clip.z0, clip.z1].  ROWS_ONLY: return the rows
+// the walk would visit instead (empty for triangles that set nothing).
+template <bool ROWS_ONLY, class Sink>
+__device__ Rows solid_triangle(const VoxParams& p, const float* t, Sink* out, const Rows clip) {
+    if (!tri_finite(t)) return {1, 0, 1, 0};
     const f3 bmin = {p.bmin[0], p.bmin[1], p.bmin[2]};
     f3 v0 = sub3({t[0], t[1], t[2]}, bmin);
     f3 v1 = sub3({t[3], t[4], t[5]}, bmin);
     f3 v2 = sub3({t[6], t[7], t[8]}, bmin);
     f3 e0 = sub3(v1, v0), e1 = sub3(v2, v1);
     f3 n = normalize3(cross3(e0, e1));
-    if ((double)fabsf(n.x) < kFloatError) return;
-    if (!(isfinite(n.x) && isfinite(n.y) && isfinite(n.z))) return;
+    if ((double)fabsf(n.x) < kFloatError) return {1, 0, 1, 0};
+    if (!(isfinite(n.x) && isfinite(n.y) && isfinite(n.z))) return {1, 0, 1, 0};
 
     f2 a = {v0.y, v0.z}, b = {v1.y, v1.z}, c = {v2.y, v2.z};
     {
@@ -245,10 +501,10 @@ __device__ void solid_triangle(const VoxParams& p, const float* t, uint32_t* tab
     int y0 = (int)fmaxf(gmin_y, -1.0f), y1 = (int)fminf(gmax_y, (float)p.G);
     int z0 = (int)fmaxf(gmin_z, -1.0f), z1 = (int)fminf(gmax_z, (float)p.G);
     y0 = max(y0, 0); z0 = max(z0, 0); y1 = min(y1, G1); z1 = min(z1, G1);
+    if (ROWS_ONLY) return {y0, y1, z0, z1};
 
-    WordWriter<true> out{table};
-    for (int y = y0; y <= y1; ++y) {
-        for (int z = z0; z <= z1; ++z) {
+    for (int y = max(y0, clip.y0); y <= min(y1, clip.y1); ++y) {
+        for (int z = max(z0, clip.z0); z <= min(z1, clip.z1); ++z) {
             const f2 pt = {(float)(((double)y + 0.5) * (double)uy), (float)(((double)z + 0.5) * (double)uz)};
             const int k = check_point_triangle(a, b, c, pt);
             if ((k == 1 && top_left_edge(a, b)) || (k == 2 && top_left_edge(b, c)) ||
@@ -258,36 +514,364 @@ __device__ void solid_triangle(const VoxParams& p, const float* t, uint32_t* tab
                 if (!(xq > -1.0)) continue;
                 const int xmax = (xq >= (double)G1) ? G1 : (int)xq;
                 if (p.morton) {
-                    for (int x = 0; x <= xmax; ++x) out.set(vox_index(p, x, y, z));
-                    out.flush();
+                    for (int x = 0; x <= xmax; ++x) out->set(vox_index(p, x, y, z), z);
+                    out->flush();
                 } else {
                     const uint64_t row = vox_index(p, 0, y, z);
-                    out.set_range(row, row + (uint64_t)xmax);
+                    out->set_range(row, row + (uint64_t)xmax, z);
                 }
             }
         }
     }
-    out.flush();
+    out->flush();
+    return {y0, y1, z0, z1};
 }
 
+constexpr int kMaxTiles = 9;            // triangles touching more tiles take the direct path
+constexpr uint32_t kTileWords = 16384;  // max LDS words of one tile (64 KiB)
+constexpr uint32_t kMaxTileCount = 16384;
+constexpr uint32_t kMinTile = 8;
+
+// Tile geometry of a row-major table: tiles of T x T rows (full x extent).
+struct TileGeo {
+    uint32_t T, tiles_y, wpp;  // wpp: LDS words per plane of a tile
+    uint32_t n_tiles;
+    uint32_t tris_per_block;   // triangles per count / scatter block
+    uint32_t chunk;            // list entries per tile workgroup
+    int aligned;               // G % 32 == 0: a tile owns whole words, rows are whole words
+    int accumulate;            // the table keeps its earlier bits
+};
+
+// Per-triangle route, written by the count pass: the tile range packed as
+// 4 x 16 bits (ty0, ty1, tz0, tz1), or one of these markers.
+constexpr uint64_t kRouteNone = ~0ull;
+constexpr uint64_t kRouteDirect = ~0ull - 1;
+
 template <bool SOLID>
-__global__ __launch_bounds__(kVoxThreads) void vox_kernel(VoxParams p, const float* __restrict__ tris,
-                                                          uint32_t* __restrict__ table) {
+__device__ __forceinline__ Rows rows_of(const VoxParams& p, const float* t) {
+    if constexpr (SOLID) return solid_triangle<true, GlobalSink<true>>(p, t, nullptr, Rows{0, -1, 0, -1});
+    else return surface_rows(p, t);
+}
+
+__device__ __forceinline__ uint64_t route_of(const Rows& r, const TileGeo& g) {
+    if (r.y0 > r.y1 || r.z0 > r.z1) return kRouteNone;
+    const uint32_t ty0 = (uint32_t)r.y0 / g.T, ty1 = (uint32_t)r.y1 / g.T;
+    const uint32_t tz0 = (uint32_t)r.z0 / g.T, tz1 = (uint32_t)r.z1 / g.T;
+    if ((ty1 - ty0 + 1) * (tz1 - tz0 + 1) > (uint32_t)kMaxTiles) return kRouteDirect;
+    return (uint64_t)ty0 | ((uint64_t)ty1 << 16) | ((uint64_t)tz0 << 32) | ((uint64_t)tz1 << 48);
+}
+
+// Calls f(tile) for every tile of a packed tile range.
+template <class Fn>
+__device__ __forceinline__ void for_tiles(uint64_t rt, const TileGeo& g, Fn&& f) {
+    const uint32_t ty0 = (uint32_t)(rt & 0xffff), ty1 = (uint32_t)((rt >> 16) & 0xffff);
+    const uint32_t tz0 = (uint32_t)((rt >> 32) & 0xffff), tz1 = (uint32_t)(rt >> 48);
+    for (uint32_t tz = tz0; tz <= tz1; ++tz)
+        for (uint32_t ty = ty0; ty <= ty1; ++ty) f(tz * g.tiles_y + ty);
+}
+
+// Coalesced copy of triangles [base, base + nt) (nt <= kVoxThreads) into LDS.
+__device__ __forceinline__ void stage_triangles(float* s_tri, const float* __restrict__ tris, uint64_t base,
+                                                uint32_t nt) {
+    const float* src = tris + base * 9;
+    for (uint32_t k = threadIdx.x; k < nt * 9; k += kVoxThreads) s_tri[k] = src[k];
+}
+
+// Direct path, solid: one triangle per lane, global atomics.
+// routes (nullable): only the triangles routed to the direct path.
+__global__ __launch_bounds__(kVoxThreads) void vox_solid_kernel(VoxParams p, const float* __restrict__ tris,
+                                                                const uint64_t* __restrict__ routes,
+                                                                uint32_t* __restrict__ table) {
     __shared__ float s_tri[kVoxThreads * 9];
+    const Rows all{0, (int)p.G - 1, 0, (int)p.G - 1};
     for (uint64_t base = (uint64_t)blockIdx.x * kVoxThreads; base < p.n_tri;
          base += (uint64_t)gridDim.x * kVoxThreads) {
         const uint32_t nt = (uint32_t)min<uint64_t>(kVoxThreads, p.n_tri - base);
-        const float* src = tris + base * 9;
-        for (uint32_t k = threadIdx.x; k < nt * 9; k += kVoxThreads) s_tri[k] = src[k];
+        if (routes) {
+            const uint64_t i = base + threadIdx.x;
+            if (threadIdx.x < nt && routes[i] == kRouteDirect) {
+                float t[9];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) t[k] = tris[i * 9 + k];
+                GlobalSink<true> out{table};
+                solid_triangle<false>(p, t, &out, all);
+            }
+            continue;
+        }
+        stage_triangles(s_tri, tris, base, nt);
         __syncthreads();
         if (threadIdx.x < nt) {
             float t[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) t[k] = s_tri[threadIdx.x * 9 + k];
-            if (SOLID) solid_triangle(p, t, table);
-            else surface_triangle(p, t, table);
+            GlobalSink<true> out{table};
+            solid_triangle<false>(p, t, &out, all);
         }
         __syncthreads();
+    }
+}
+
+// Direct path, surface: one triangle per lane, swept along x so that the
+// bits of one row merge into one global atomic per word.  routes (nullable):
+// only the triangles routed to the direct path.
+__global__ __launch_bounds__(kVoxThreads) void vox_surface_kernel(VoxParams p, const float* __restrict__ tris,
+                                                                  const uint64_t* __restrict__ routes,
+                                                                  uint32_t* __restrict__ table) {
+    __shared__ float s_tri[kVoxThreads * 9];
+    const Rows all{0, (int)p.G - 1, 0, (int)p.G - 1};
+    for (uint64_t base = (uint64_t)blockIdx.x * kVoxThreads; base < p.n_tri;
+         base += (uint64_t)gridDim.x * kVoxThreads) {
+        const uint32_t nt = (uint32_t)min<uint64_t>(kVoxThreads, p.n_tri - base);
+        float t[9];
+        bool have = threadIdx.x < nt;
+        if (routes) {
+            have = have && routes[base + threadIdx.x] == kRouteDirect;
+            if (have) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) t[k] = tris[(base + threadIdx.x) * 9 + k];
+            }
+        } else {
+            stage_triangles(s_tri, tris, base, nt);
+            __syncthreads();
+            if (have) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) t[k] = s_tri[threadIdx.x * 9 + k];
+            }
+            __syncthreads();
+        }
+        if (!have) continue;
+        const SurfTri st = surface_setup(p, t, all, true);
+        if (st.axis != 0) continue;
+        GlobalSink<false> out{table};
+        const uint32_t cols = surface_columns<0>(st);
+        for (uint32_t j = 0; j < cols; ++j) surface_column<0>(p, st, j, out);
+        out.flush();
+    }
+}
+
+// ---- tiled path
+
+// counts[tile][block] = list entries of this block's triangles in the tile;
+// routes[i] = triangle i's packed tile range or marker.
+// LDS: n_tiles counters, then the triangle staging buffer.
+template <bool SOLID>
+__global__ __launch_bounds__(kVoxThreads) void tile_count_kernel(VoxParams p, const float* __restrict__ tris,
+                                                                 TileGeo geo, uint32_t n_blocks,
+                                                                 uint32_t* __restrict__ counts,
+                                                                 uint64_t* __restrict__ routes) {
+    extern __shared__ uint32_t cnt[];
+    float* s_tri = reinterpret_cast<float*>(cnt + geo.n_tiles);
+    for (uint32_t t = threadIdx.x; t < geo.n_tiles; t += kVoxThreads) cnt[t] = 0;
+    const uint64_t b0 = (uint64_t)blockIdx.x * geo.tris_per_block;
+    const uint64_t b1 = min<uint64_t>(p.n_tri, b0 + geo.tris_per_block);
+    for (uint64_t base = b0; base < b1; base += kVoxThreads) {
+        const uint32_t nt = (uint32_t)min<uint64_t>(kVoxThreads, b1 - base);
+        __syncthreads();
+        stage_triangles(s_tri, tris, base, nt);
+        __syncthreads();
+        if (threadIdx.x < nt) {
+            float t[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) t[k] = s_tri[threadIdx.x * 9 + k];
+            const uint64_t rt = route_of(rows_of<SOLID>(p, t), geo);
+            routes[base + threadIdx.x] = rt;
+            if (rt < kRouteDirect) for_tiles(rt, geo, [&](uint32_t tile) { atomicAdd(&cnt[tile], 1u); });
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < geo.n_tiles; t += kVoxThreads)
+        counts[(size_t)t * n_blocks + blockIdx.x] = cnt[t];
+}
+
+// Per tile (one block each): exclusive scan of its row of block counts, in
+// place, and the row total.
+__global__ __launch_bounds__(kVoxThreads) void tile_scan_rows_kernel(uint32_t n_blocks, uint32_t* __restrict__ counts,
+                                                                     uint32_t* __restrict__ totals) {
+    __shared__ uint32_t part[kVoxThreads];
+    uint32_t* row = counts + (size_t)blockIdx.x * n_blocks;
+    const uint32_t per = (n_blocks + kVoxThreads - 1) / kVoxThreads;
+    const uint32_t b0 = min(n_blocks, threadIdx.x * per), b1 = min(n_blocks, b0 + per);
+    uint32_t s = 0;
+    for (uint32_t i = b0; i < b1; ++i) s += row[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < kVoxThreads; ++i) {
+            const uint32_t v = part[i];
+            part[i] = acc;
+            acc += v;
+        }
+        totals[blockIdx.x] = acc;
+    }
+    __syncthreads();
+    uint32_t acc = part[threadIdx.x];
+    for (uint32_t i = b0; i < b1; ++i) {
+        const uint32_t v = row[i];
+        row[i] = acc;
+        acc += v;
+    }
+}
+
+// One block: first slot of each tile's list, chunk numbering (chunk_start has
+// n_tiles + 1 entries; the last is the number of chunks).
+__global__ __launch_bounds__(kVoxThreads) void tile_scan_kernel(TileGeo geo, const uint32_t* __restrict__ totals,
+                                                                uint32_t* __restrict__ start,
+                                                                uint32_t* __restrict__ chunk_start) {
+    __shared__ uint32_t pr[kVoxThreads], pc[kVoxThreads];
+    const uint32_t n = geo.n_tiles;
+    const uint32_t per = (n + kVoxThreads - 1) / kVoxThreads;
+    const uint32_t b0 = min(n, threadIdx.x * per), b1 = min(n, b0 + per);
+    uint32_t sr = 0, sc = 0;
+    for (uint32_t i = b0; i < b1; ++i) {
+        sr += totals[i];
+        sc += (totals[i] + geo.chunk - 1) / geo.chunk;
+    }
+    pr[threadIdx.x] = sr;
+    pc[threadIdx.x] = sc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ar = 0, ac = 0;
+        for (int i = 0; i < kVoxThreads; ++i) {
+            const uint32_t vr = pr[i], vc = pc[i];
+            pr[i] = ar;
+            pc[i] = ac;
+            ar += vr;
+            ac += vc;
+        }
+        chunk_start[n] = ac;
+    }
+    __syncthreads();
+    uint32_t ar = pr[threadIdx.x], ac = pc[threadIdx.x];
+    for (uint32_t i = b0; i < b1; ++i) {
+        start[i] = ar;
+        chunk_start[i] = ac;
+        ar += totals[i];
+        ac += (totals[i] + geo.chunk - 1) / geo.chunk;
+    }
+}
+
+// Writes the index of every tiled triangle into the lists of its tiles (in
+// the same block partition as the count pass).
+__global__ __launch_bounds__(kVoxThreads) void tile_scatter_kernel(uint64_t n_tri, TileGeo geo, uint32_t n_blocks,
+                                                                   const uint64_t* __restrict__ routes,
+                                                                   const uint32_t* __restrict__ row_off,
+                                                                   const uint32_t* __restrict__ start,
+                                                                   uint32_t* __restrict__ idx) {
+    extern __shared__ uint32_t cursor[];
+    for (uint32_t t = threadIdx.x; t < geo.n_tiles; t += kVoxThreads)
+        cursor[t] = start[t] + row_off[(size_t)t * n_blocks + blockIdx.x];
+    __syncthreads();
+    const uint64_t b0 = (uint64_t)blockIdx.x * geo.tris_per_block;
+    const uint64_t b1 = min<uint64_t>(n_tri, b0 + geo.tris_per_block);
+    for (uint64_t i = b0 + threadIdx.x; i < b1; i += kVoxThreads) {
+        const uint64_t rt = routes[i];
+        if (rt < kRouteDirect)
+            for_tiles(rt, geo, [&](uint32_t tile) { idx[atomicAdd(&cursor[tile], 1u)] = (uint32_t)i; });
+    }
+}
+
+// Suffix-XOR along each word-aligned row of the LDS tile (solid DIFF form):
+// bit x of a row becomes the XOR of the toggles at x' >= x.  Bit 31-b of a
+// word is x = 32w + b, so within a word the suffix runs from bit 0 upwards.
+__device__ __forceinline__ void rows_suffix_xor(uint32_t* lds, uint32_t n_rows, uint32_t row_words) {
+    for (uint32_t r = threadIdx.x; r < n_rows; r += kVoxThreads) {
+        uint32_t* row = lds + r * row_words;
+        uint32_t carry = 0;
+        for (int w = (int)row_words - 1; w >= 0; --w) {
+            uint32_t v = row[w];
+            v ^= v << 1;
+            v ^= v << 2;
+            v ^= v << 4;
+            v ^= v << 8;
+            v ^= v << 16;
+            if (carry) v = ~v;
+            carry = v >> 31;
+            row[w] = v;
+        }
+    }
+}
+
+// Persistent: each iteration takes one chunk of one tile's list, runs the
+// per-triangle routine clipped to the tile's rows into an LDS copy of the
+// tile, then merges the nonzero words into the table.
+template <bool SOLID, bool DIFF>
+__global__ __launch_bounds__(kVoxThreads) void tile_vox_kernel(VoxParams p, const float* __restrict__ tris,
+                                                               TileGeo geo, const uint32_t* __restrict__ totals,
+                                                               const uint32_t* __restrict__ start,
+                                                               const uint32_t* __restrict__ chunk_start,
+                                                               const uint32_t* __restrict__ idx,
+                                                               uint32_t* __restrict__ table) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t n_chunks = chunk_start[geo.n_tiles];
+    const uint32_t tile_words = geo.T * geo.wpp;
+    const uint64_t GG = (uint64_t)p.G * p.G;
+    for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        uint32_t lo = 0, hi = geo.n_tiles;  // invariant: chunk_start[lo] <= c < chunk_start[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (chunk_start[mid] <= c) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t tile = lo;
+        const int ty = (int)(tile % geo.tiles_y), tz = (int)(tile / geo.tiles_y);
+        const Rows clip{ty * (int)geo.T, min((ty + 1) * (int)geo.T, (int)p.G) - 1, tz * (int)geo.T,
+                        min((tz + 1) * (int)geo.T, (int)p.G) - 1};
+        const uint32_t begin = start[tile] + (c - chunk_start[tile]) * geo.chunk;
+        const uint32_t end = min(start[tile] + totals[tile], begin + geo.chunk);
+        for (uint32_t k = threadIdx.x; k < tile_words; k += kVoxThreads) lds[k] = 0;
+        __syncthreads();
+        if constexpr (SOLID) {
+            for (uint32_t k = begin + threadIdx.x; k < end; k += kVoxThreads) {
+                const uint64_t i = idx[k];
+                float t[9];
+#pragma unroll
+                for (int j = 0; j < 9; ++j) t[j] = tris[i * 9 + j];
+                LdsSink<true, DIFF> out{lds, (uint64_t)p.G, clip.z0, clip.y0, geo.wpp};
+                solid_triangle<false>(p, t, &out, clip);
+            }
+        } else {
+            // after the tile: set-up triangles, column prefix, scan partials
+            SurfTri* s_st = reinterpret_cast<SurfTri*>(lds + tile_words);
+            uint32_t* s_pre = reinterpret_cast<uint32_t*>(s_st + kVoxThreads);
+            uint32_t* s_part = s_pre + kVoxThreads + 1;
+            for (uint32_t kb = begin; kb < end; kb += kVoxThreads) {
+                const uint32_t k = kb + threadIdx.x;
+                SurfTri st;
+                st.axis = -1;
+                if (k < end) {
+                    const uint64_t i = idx[k];
+                    float t[9];
+#pragma unroll
+                    for (int j = 0; j < 9; ++j) t[j] = tris[i * 9 + j];
+                    st = surface_setup(p, t, clip);
+                }
+                s_st[threadIdx.x] = st;
+                __syncthreads();
+                LdsSink<false> out{lds, (uint64_t)p.G, clip.z0, clip.y0, geo.wpp};
+                surface_block(p, s_st, s_pre, s_part, out);
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+        if constexpr (DIFF) {
+            // rows of plane zz start at zz * wpp, each G / 32 words (aligned G)
+            rows_suffix_xor(lds, geo.T * geo.T, p.G / 32);
+            __syncthreads();
+        }
+        // a tile with one chunk that owns whole words is the only writer of them
+        const bool exclusive = geo.aligned && chunk_start[tile + 1] - chunk_start[tile] == 1;
+        const uint32_t planes = (uint32_t)(clip.z1 - clip.z0 + 1);
+        for (uint32_t k = threadIdx.x; k < planes * geo.wpp; k += kVoxThreads) {
+            const uint32_t v = lds[k];
+            if (!v) continue;
+            const uint32_t zz = k / geo.wpp;
+            const uint64_t w = (((uint64_t)(clip.z0 + (int)zz) * GG + (uint64_t)clip.y0 * p.G) >> 5) + (k - zz * geo.wpp);
+            if (exclusive) table[w] = geo.accumulate ? (SOLID ? (table[w] ^ v) : (table[w] | v)) : v;
+            else if (SOLID) atomicXor(table + w, v);
+            else atomicOr(table + w, v);
+        }
+        __syncthreads();  // LDS is reused by the next chunk
     }
 }
 
@@ -367,6 +951,112 @@ uint32_t* bbox_accumulator() {
     return per_dev[dev];
 }
 
+// Tile shape for a G^3 row-major table: T x T rows with T the smallest
+// size >= kMinTile that keeps the tile count within kMaxTileCount (the count
+// pass keeps one LDS counter per tile).  False when such a tile does not fit
+// kTileWords LDS words, or (unless forced) for small jobs, where the extra
+// passes cost more than the global atomics they save.
+bool tile_geometry(uint32_t G, uint64_t n_tri, bool force, bool accumulate, TileGeo& g) {
+    if ((!force && n_tri < 4096) || n_tri * kMaxTiles >= (1ull << 32)) return false;
+    const bool aligned = (G % 32) == 0;
+    uint64_t T = std::min<uint64_t>(G, kMinTile);
+    while ((G + T - 1) / T * ((G + T - 1) / T) > kMaxTileCount) ++T;
+    const uint64_t wpp = aligned ? T * G / 32 : (T * G + 31) / 32 + 1;
+    if (T * wpp > kTileWords) return false;
+    const uint64_t tiles_y = (G + T - 1) / T;
+    g.T = (uint32_t)T;
+    g.tiles_y = (uint32_t)tiles_y;
+    g.wpp = (uint32_t)wpp;
+    g.n_tiles = (uint32_t)(tiles_y * tiles_y);
+    // at most ~1024 count / scatter blocks, each a multiple of the staging size
+    const uint64_t per = std::max<uint64_t>(2048, (n_tri + 1023) / 1024);
+    g.tris_per_block = (uint32_t)((per + kVoxThreads - 1) / kVoxThreads * kVoxThreads);
+    // list entries per workgroup: about 16 chunks per CU for ~2 entries per triangle
+    const uint64_t chunk = 2 * n_tri / (16ull * (uint64_t)cu_count());
+    g.chunk = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(chunk, 512), 1u << 16);
+    g.aligned = aligned ? 1 : 0;
+    g.accumulate = accumulate ? 1 : 0;
+    return true;
+}
+
+// Tiled-path scratch, one grow-only buffer per (device, stream).
+uint8_t* tile_workspace(hipStream_t s, size_t bytes) {
+    static std::mutex mu;
+    static std::map<std::pair<int, hipStream_t>, std::unique_ptr<DevBuf>> per;
+    int dev = 0;
+    NLOD_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto& b = per[std::make_pair(dev, s)];
+    if (!b) b.reset(new DevBuf());
+    if (b->bytes < bytes) {
+        NLOD_HIP(hipStreamSynchronize(s));  // earlier calls on this stream may still read it
+        b->ensure(bytes);
+    }
+    return b->as<uint8_t>();
+}
+
+template <bool SOLID>
+void launch_direct(hipStream_t stream, const VoxParams& p, const float* d_tris, const uint64_t* routes,
+                   uint32_t* d_table) {
+    const uint64_t blocks = std::min<uint64_t>((p.n_tri + kVoxThreads - 1) / kVoxThreads, 1u << 20);
+    if (SOLID)
+        hipLaunchKernelGGL(vox_solid_kernel, dim3((uint32_t)blocks), dim3(kVoxThreads), 0, stream, p, d_tris, routes,
+                           d_table);
+    else
+        hipLaunchKernelGGL(vox_surface_kernel, dim3((uint32_t)blocks), dim3(kVoxThreads), 0, stream, p, d_tris,
+                           routes, d_table);
+    NLOD_LAUNCH_CHECK();
+}
+
+template <bool SOLID>
+void launch_voxelize(hipStream_t stream, const VoxParams& p, const TileGeo& geo, bool tiled, const float* d_tris,
+                     uint32_t* d_table) {
+    if (!tiled) {
+        launch_direct<SOLID>(stream, p, d_tris, nullptr, d_table);
+        return;
+    }
+    const uint32_t n_blocks = (uint32_t)((p.n_tri + geo.tris_per_block - 1) / geo.tris_per_block);
+    const size_t n_counts = (size_t)geo.n_tiles * n_blocks;
+    const size_t n_idx = (size_t)p.n_tri * kMaxTiles;
+    // routes (u64 per triangle) | counts | totals | start | chunk_start (n_tiles + 1) | idx
+    const size_t words = 2 * (size_t)p.n_tri + n_counts + 3 * (size_t)geo.n_tiles + 1 + n_idx;
+    uint8_t* ws = tile_workspace(stream, words * sizeof(uint32_t));
+    uint64_t* routes = reinterpret_cast<uint64_t*>(ws);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(routes + p.n_tri);
+    uint32_t* totals = counts + n_counts;
+    uint32_t* start = totals + geo.n_tiles;
+    uint32_t* chunk_start = start + geo.n_tiles;
+    uint32_t* idx = chunk_start + geo.n_tiles + 1;
+    const size_t count_smem = (size_t)geo.n_tiles * 4 + kVoxThreads * 9 * sizeof(float);
+    const size_t scatter_smem = (size_t)geo.n_tiles * 4;
+    (void)occupancy_blocks(reinterpret_cast<const void*>(tile_count_kernel<SOLID>), kVoxThreads, count_smem);
+    (void)occupancy_blocks(reinterpret_cast<const void*>(tile_scatter_kernel), kVoxThreads, scatter_smem);
+    hipLaunchKernelGGL(tile_count_kernel<SOLID>, dim3(n_blocks), dim3(kVoxThreads), count_smem, stream, p, d_tris,
+                       geo, n_blocks, counts, routes);
+    NLOD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tile_scan_rows_kernel, dim3(geo.n_tiles), dim3(kVoxThreads), 0, stream, n_blocks, counts,
+                       totals);
+    NLOD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(kVoxThreads), 0, stream, geo, totals, start, chunk_start);
+    NLOD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(tile_scatter_kernel, dim3(n_blocks), dim3(kVoxThreads), scatter_smem, stream, p.n_tri, geo,
+                       n_blocks, routes, counts, start, idx);
+    NLOD_LAUNCH_CHECK();
+    size_t tile_smem = (size_t)geo.T * geo.wpp * sizeof(uint32_t);
+    if (!SOLID) tile_smem += kVoxThreads * sizeof(SurfTri) + (kVoxThreads + 1 + kVoxThreads / 64) * sizeof(uint32_t);
+    auto run_tiles = [&](auto kernel) {
+        const int per_cu = occupancy_blocks(reinterpret_cast<const void*>(kernel), kVoxThreads, tile_smem);
+        const uint32_t grid = (uint32_t)(cu_count() * per_cu);
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kVoxThreads), tile_smem, stream, p, d_tris, geo, totals, start,
+                           chunk_start, idx, d_table);
+        NLOD_LAUNCH_CHECK();
+    };
+    if (SOLID && geo.aligned) run_tiles(tile_vox_kernel<SOLID, true>);
+    else run_tiles(tile_vox_kernel<SOLID, false>);
+    // the triangles spanning more than kMaxTiles tiles
+    launch_direct<SOLID>(stream, p, d_tris, routes, d_table);
+}
+
 void validate_info(const nlod_voxinfo* info, uint32_t flags) {
     NLOD_CHECK(info != nullptr, "voxinfo is null");
     const uint32_t G = info->gridsize[0];
@@ -374,6 +1064,8 @@ void validate_info(const nlod_voxinfo* info, uint32_t flags) {
                "only cubic grids are supported (gridsize.x == gridsize.y == gridsize.z > 0)");
     NLOD_CHECK(G <= 65536, "grid resolution above 65536 is not supported");
     if (flags & NLOD_VOX_MORTON) NLOD_CHECK((G & (G - 1)) == 0, "Morton order needs a power-of-two grid");
+    NLOD_CHECK((flags & (NLOD_VOX_DIRECT | NLOD_VOX_TILED)) != (NLOD_VOX_DIRECT | NLOD_VOX_TILED),
+               "NLOD_VOX_DIRECT and NLOD_VOX_TILED exclude each other");
     for (int i = 0; i < 3; ++i)
         NLOD_CHECK(std::isfinite(info->unit[i]) && info->unit[i] > 0.0f && std::isfinite(info->bbox_min[i]),
                    "voxinfo has a non-positive or non-finite unit / box");
@@ -432,12 +1124,16 @@ void voxelize(hipStream_t stream, const nlod_voxinfo* info, const float* d_tris,
     p.G = (uint32_t)G;
     p.morton = (flags & NLOD_VOX_MORTON) ? 1 : 0;
     p.n_tri = info->n_triangles;
-    const uint64_t blocks = std::min<uint64_t>((p.n_tri + kVoxThreads - 1) / kVoxThreads, 1u << 20);
-    if (flags & NLOD_VOX_SOLID)
-        hipLaunchKernelGGL(vox_kernel<true>, dim3((uint32_t)blocks), dim3(kVoxThreads), 0, stream, p, d_tris, d_table);
-    else
-        hipLaunchKernelGGL(vox_kernel<false>, dim3((uint32_t)blocks), dim3(kVoxThreads), 0, stream, p, d_tris, d_table);
-    NLOD_LAUNCH_CHECK();
+    TileGeo geo{};
+    // default: tiled for the solid fill (its row ranges make global atomics
+    // the bound of the direct kernel); the surface test is compute-bound
+    // either way and the direct kernel has no binning passes
+    const bool want = (flags & NLOD_VOX_TILED) || ((flags & NLOD_VOX_SOLID) && !(flags & NLOD_VOX_DIRECT));
+    const bool tiled = !p.morton && want &&
+                       tile_geometry(p.G, p.n_tri, (flags & NLOD_VOX_TILED) != 0, (flags & NLOD_VOX_ACCUMULATE) != 0,
+                                     geo);
+    if (flags & NLOD_VOX_SOLID) launch_voxelize<true>(stream, p, geo, tiled, d_tris, d_table);
+    else launch_voxelize<false>(stream, p, geo, tiled, d_tris, d_table);
 }
 
 void mesh_bbox(hipStream_t stream, const float* d_tris, uint64_t n_tri, float bmin[3], float bmax[3]) {

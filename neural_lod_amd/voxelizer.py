@@ -21,11 +21,16 @@ SOLID = 1 << 0
 MORTON = 1 << 1
 PAD_EPS = 1 << 2
 ACCUMULATE = 1 << 3
+DIRECT = 1 << 4  # one triangle per lane, global atomics
+TILED = 1 << 5   # LDS-tiled whenever the table allows it
+_PATHS = {"auto": 0, "direct": DIRECT, "tiled": TILED}
 
 
-def _flags(solid: bool, pad: bool, morton: bool, accumulate: bool = False) -> int:
+def _flags(solid: bool, pad: bool, morton: bool, accumulate: bool = False, path: str = "auto") -> int:
+    if path not in _PATHS:
+        raise ValueError(f"path must be one of {sorted(_PATHS)}")
     return (SOLID if solid else 0) | (PAD_EPS if pad else 0) | (MORTON if morton else 0) | (
-        ACCUMULATE if accumulate else 0)
+        ACCUMULATE if accumulate else 0) | _PATHS[path]
 
 
 def table_words(res: int) -> int:
@@ -63,11 +68,15 @@ def mesh_bbox(triangles: torch.Tensor) -> tuple[np.ndarray, np.ndarray]:
 
 
 def voxelize(triangles: torch.Tensor, res: int, solid: bool = False, pad: bool = False, morton: bool = False,
-             info: VoxInfo | None = None, out: torch.Tensor | None = None) -> tuple[torch.Tensor, VoxInfo]:
+             info: VoxInfo | None = None, out: torch.Tensor | None = None,
+             path: str = "auto") -> tuple[torch.Tensor, VoxInfo]:
     """Voxelize a device triangle soup; returns (int32 bit table on device, voxinfo).
 
     The table holds ceil(res^3/32) 32-bit words (reinterpret as uint32).
-    With ``info`` given, that box is used as is (no bounding-box pass)."""
+    With ``info`` given, that box is used as is (no bounding-box pass).
+    ``path`` picks the kernels ("direct": one triangle per lane with global
+    atomics; "tiled": LDS tiles of voxel rows; "auto": tiled for larger
+    row-major jobs); all give identical bits."""
     t = _check_tris(triangles)
     words = table_words(res)
     if out is None:
@@ -75,7 +84,7 @@ def voxelize(triangles: torch.Tensor, res: int, solid: bool = False, pad: bool =
     elif out.numel() < words or out.dtype != torch.int32 or not out.is_contiguous():
         raise ValueError(f"out must be a contiguous int32 tensor of at least {words} words")
     stream = C.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
-    flags = _flags(solid, pad, morton)
+    flags = _flags(solid, pad, morton, path=path)
     if info is None:
         info = VoxInfo()
         _lib.call("nlod_voxelize_mesh", stream, C.c_void_p(t.data_ptr()), t.shape[0], res, flags,

@@ -1,4 +1,8 @@
-"""GPU voxelizer (libnlod nlod_voxelize*) vs the CPU oracle: bit-exact tables."""
+"""GPU voxelizer (libnlod nlod_voxelize*) vs the CPU oracle: bit-exact tables.
+
+Every case runs the default kernel choice and both kernel paths (direct:
+one triangle per lane with global atomics; tiled: LDS tiles of voxel rows,
+with triangles spanning many tiles handed to the direct kernel)."""
 import numpy as np
 import pytest
 import torch
@@ -21,16 +25,22 @@ def _assert_same(gpu, cpu, what):
     assert diff.size == 0, f"{what}: {diff.size} words differ, first at {diff[:5]}"
 
 
+def _check_paths(tris, res, cpu, what, **kw):
+    for path in ("auto", "direct", "tiled"):
+        gpu, info = _gpu_table(tris, res, path=path, **kw)
+        _assert_same(gpu, cpu, f"{what} path={path}")
+    return info
+
+
 @pytest.mark.parametrize("solid", [False, True])
 @pytest.mark.parametrize("pad", [False, True])
 def test_blob_mesh_200k_256(oracle, cuda, solid, pad):
     tris = S.blob_mesh(200_000, seed=0)
     print(f"case blob200k G=256 solid={solid} pad={pad}", flush=True)
     cpu, cinfo = oracle.voxelize(tris, 256, solid=solid, pad=pad)
-    gpu, ginfo = _gpu_table(tris, 256, solid=solid, pad=pad)
+    ginfo = _check_paths(tris, 256, cpu, f"blob solid={solid} pad={pad}", solid=solid, pad=pad)
     assert list(ginfo.unit) == list(cinfo.unit)
     assert list(ginfo.bbox_min) == list(cinfo.bbox_min)
-    _assert_same(gpu, cpu, f"blob solid={solid} pad={pad}")
 
 
 @pytest.mark.parametrize("res", [64, 128, 256, 512, 100])
@@ -39,8 +49,7 @@ def test_bunny(oracle, cuda, golden_dir, res, solid):
     tris = np.load(golden_dir / "bunny_tris.npy")
     print(f"case bunny G={res} solid={solid}", flush=True)
     cpu, _ = oracle.voxelize(tris, res, solid=solid)
-    gpu, _ = _gpu_table(tris, res, solid=solid)
-    _assert_same(gpu, cpu, f"bunny res={res} solid={solid}")
+    _check_paths(tris, res, cpu, f"bunny res={res} solid={solid}", solid=solid)
 
 
 @pytest.mark.parametrize("solid", [False, True])
@@ -56,8 +65,7 @@ def test_closed_shapes(oracle, cuda):
         for res in (48, 64, 257):
             for solid in (False, True):
                 cpu, _ = oracle.voxelize(tris, res, solid=solid)
-                gpu, _ = _gpu_table(tris, res, solid=solid)
-                _assert_same(gpu, cpu, f"closed res={res} solid={solid}")
+                _check_paths(tris, res, cpu, f"closed res={res} solid={solid}", solid=solid)
 
 
 def test_edge_cases(oracle, cuda):
@@ -76,8 +84,19 @@ def test_edge_cases(oracle, cuda):
             for solid in (False, True):
                 for pad in (False, True):
                     cpu, _ = oracle.voxelize(tris, res, solid=solid, pad=pad)
-                    gpu, _ = _gpu_table(tris, res, solid=solid, pad=pad)
-                    _assert_same(gpu, cpu, f"{name} res={res} solid={solid} pad={pad}")
+                    _check_paths(tris, res, cpu, f"{name} res={res} solid={solid} pad={pad}", solid=solid,
+                                 pad=pad)
+
+
+@pytest.mark.parametrize("scale", [1e-15, 1e-6, 1e6])
+def test_scaled_meshes(oracle, cuda, scale):
+    """Tiny and large coordinates (the plane-slab pruning of the surface test
+    switches itself off below its underflow-safe range)."""
+    tris = (S.blob_mesh(5000, seed=4).astype(np.float64) * scale + 3.0 * scale).astype(np.float32)
+    for res in (64, 96):
+        for solid in (False, True):
+            cpu, _ = oracle.voxelize(tris, res, solid=solid)
+            _check_paths(tris, res, cpu, f"scale={scale} res={res} solid={solid}", solid=solid)
 
 
 def test_non_finite_and_outside_box(oracle, cuda):
@@ -91,8 +110,19 @@ def test_non_finite_and_outside_box(oracle, cuda):
         info_c = oracle.voxinfo([0, 0, 0], [1, 1, 1], 32, False, tris.shape[0])
         cpu, _ = oracle.voxelize(tris, 32, solid=solid, info=info_c)
         info_g = V.voxinfo_from_bbox([0, 0, 0], [1, 1, 1], 32)
-        gpu, _ = _gpu_table(tris, 32, solid=solid, info=info_g)
-        _assert_same(gpu, cpu, f"outside-box solid={solid}")
+        _check_paths(tris, 32, cpu, f"outside-box solid={solid}", solid=solid, info=info_g)
+
+
+@pytest.mark.parametrize("res", [96, 256, 333])
+@pytest.mark.parametrize("solid", [False, True])
+def test_mixed_triangle_sizes(oracle, cuda, res, solid):
+    """Small triangles (tiled) mixed with ones spanning many tiles (direct),
+    on aligned (96, 256) and unaligned (333) row lengths."""
+    big = np.concatenate([S.box_mesh((0.05, 0.05, 0.05), (0.95, 0.95, 0.95)), S.sphere_mesh(6, 8)])
+    tris = np.concatenate([S.blob_mesh(20_000, seed=2), big]).astype(np.float32)
+    print(f"case mixed G={res} solid={solid}", flush=True)
+    cpu, _ = oracle.voxelize(tris, res, solid=solid)
+    _check_paths(tris, res, cpu, f"mixed res={res} solid={solid}", solid=solid)
 
 
 def test_empty_mesh_and_accumulate(cuda):
@@ -105,15 +135,16 @@ def test_empty_mesh_and_accumulate(cuda):
     assert int(out.abs().sum()) == 0
     tri = torch.tensor([[0.1, 0.1, 0.1, 0.9, 0.1, 0.1, 0.1, 0.9, 0.9]], device="cuda")
     a, _ = V.voxelize(tri, 64, info=info)
-    b = a.clone()
     tri2 = torch.tensor([[0.5, 0.5, 0.9, 0.9, 0.5, 0.1, 0.5, 0.9, 0.2]], device="cuda")
     c, _ = V.voxelize(tri2, 64, info=info)
-    from neural_lod_amd.voxelizer import _lib, _flags
+    from neural_lod_amd.voxelizer import _flags
     import ctypes as C
-    V._lib.call("nlod_voxelize", C.c_void_p(torch.cuda.current_stream().cuda_stream), C.byref(info),
-                C.c_void_p(tri2.data_ptr()), C.c_void_p(b.data_ptr()), _flags(False, False, False, True))
-    torch.cuda.synchronize()
-    assert torch.equal(b, a | c)
+    for path in ("direct", "tiled"):
+        b = a.clone()
+        V._lib.call("nlod_voxelize", C.c_void_p(torch.cuda.current_stream().cuda_stream), C.byref(info),
+                    C.c_void_p(tri2.data_ptr()), C.c_void_p(b.data_ptr()), _flags(False, False, False, True, path))
+        torch.cuda.synchronize()
+        assert torch.equal(b, a | c), path
 
 
 def test_invalid_arguments(cuda):
@@ -124,3 +155,5 @@ def test_invalid_arguments(cuda):
         V.voxelize(tri, 100, morton=True)
     with pytest.raises(NlodError):
         V.voxelize(tri, 0)
+    with pytest.raises(ValueError):
+        V.voxelize(tri, 64, path="fast")
