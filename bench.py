@@ -125,6 +125,75 @@ def wavefront_bench(dev, args):
     return res
 
 
+VOX_RES = 256
+VOX_TRIS = 200_000       # the CPU-runnable voxelization workload (blob mesh, 256^3 solid)
+VOX_TRIS_LARGE = 20_000_000
+
+
+def voxelize_bench(dev, args):
+    """Solid and surface voxelization of the seeded 200k-triangle blob mesh at
+    256^3 (bit-exact check against the CPU restatement, which is also timed as
+    the CPU baseline), plus a 20M-triangle mesh of the same construction for
+    the streaming rate.  Bytes per mesh: 36 B per triangle read + the table
+    written once."""
+    import ctypes as C
+    import torch
+    from neural_lod_amd import synthetic, voxelizer as V
+    out = {"res": VOX_RES, "mesh": "blob mesh (64 Gaussian blobs, sigma 0.05, vertex offsets U[-0.01, 0.01])"}
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def time_vox(tris, solid, reps):
+        table, info = V.voxelize(tris, VOX_RES, solid=solid)
+        fl = V._flags(solid, False, False)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            V._lib.call("nlod_voxelize", stream, C.byref(info), C.c_void_p(tris.data_ptr()),
+                        C.c_void_p(table.data_ptr()), fl)
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, table
+
+    table_bytes = VOX_RES ** 3 // 8
+    tris_np = synthetic.blob_mesh(VOX_TRIS, seed=0)
+    tris = torch.from_numpy(tris_np).to(dev)
+    for mode, solid in (("solid", True), ("surface", False)):
+        ms, table = time_vox(tris, solid, max(args.steps, 20))
+        nbytes = VOX_TRIS * 36 + table_bytes
+        out[mode] = {"us_per_mesh": ms * 1e3, "triangles_per_s": VOX_TRIS / (ms * 1e-3),
+                     "achieved_GBs": nbytes / (ms * 1e-3) / 1e9, "frac_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        gpu_table = table.cpu().numpy().view(np.uint32)
+        if not args.no_cpu:
+            from oracle import oracle as O
+            t0 = time.perf_counter()
+            reps = 0
+            while True:
+                cpu_table, _ = O.voxelize(tris_np, VOX_RES, solid=solid, threads=1)
+                reps += 1
+                if time.perf_counter() - t0 > (4.0 if solid else 2.0):
+                    break
+            dt = (time.perf_counter() - t0) / reps
+            out[mode]["bit_exact_vs_cpu"] = bool(np.array_equal(gpu_table, cpu_table))
+            out[mode]["cpu_baseline"] = {"value": VOX_TRIS / dt, "unit": "triangles/s", "cores": 1, "kind": "port",
+                                         "sample": f"{reps} x the same 200k-triangle mesh at {VOX_RES}^3 "
+                                                   "(C restatement of cpu_voxelizer.cpp, one thread)"}
+        log(f"voxelize {mode} 200k @ {VOX_RES}^3: {ms * 1e3:.1f} us  {out[mode]}")
+    del tris
+    big = torch.from_numpy(synthetic.blob_mesh(VOX_TRIS_LARGE, seed=0)).to(dev)
+    for mode, solid in (("solid", True), ("surface", False)):
+        ms, _ = time_vox(big, solid, 3)
+        nbytes = VOX_TRIS_LARGE * 36 + table_bytes
+        out[mode + "_20M"] = {"ms_per_mesh": ms, "triangles_per_s": VOX_TRIS_LARGE / (ms * 1e-3),
+                              "achieved_GBs": nbytes / (ms * 1e-3) / 1e9,
+                              "frac_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        log(f"voxelize {mode} 20M @ {VOX_RES}^3: {ms:.3f} ms")
+    del big
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +203,7 @@ def main():
     ap.add_argument("--no-train", action="store_true", help="skip the training-step measurement")
     ap.add_argument("--train-steps", type=int, default=50)
     ap.add_argument("--no-wavefront", action="store_true", help="skip the wavefront-batch measurement")
+    ap.add_argument("--no-voxelize", action="store_true", help="skip the voxelization measurement")
     ap.add_argument("--profile-encode", action="store_true",
                     help="only run the FP16 encoding pass and a 256 MiB device copy (for counter profiles)")
     args = ap.parse_args()
@@ -260,8 +330,28 @@ def main():
         train["speedup_fp16_vs_fp32"] = train["fp32"]["ms_per_step"] / train["fp16"]["ms_per_step"]
         train["batch"] = nb
         train["labels"] = "analytic sphere occlusion of random rays (synthetic)"
+        if not args.no_cpu:
+            # the oracle's training step (FP64 forward/backward, float32 Adam; OpenMP
+            # over queries) on 2^14-query batches of the same workload
+            n_c = 1 << 14
+            xc = data[0][0][:n_c].cpu().numpy()
+            yc = data[0][1][:n_c].cpu().numpy()
+            p0 = M.xavier_uniform(L, seed=1337)
+            M.train(L, p0, [(xc, yc)], adam, "BinaryCrossEntropy")  # warm threads
+            t0 = time.perf_counter()
+            reps = 0
+            while time.perf_counter() - t0 < 8.0:
+                M.train(L, p0, [(xc, yc)] * 2, adam, "BinaryCrossEntropy")
+                reps += 2
+            dt = (time.perf_counter() - t0) / reps
+            train["cpu_baseline"] = {"value": n_c / dt / nb, "unit": f"steps/s at batch {nb}",
+                                     "cores": int(M._lib().nlo_omp_threads()), "kind": "port",
+                                     "sample": f"{reps} steps of {n_c} queries (FP64 C forward/backward + Adam), "
+                                               "scaled to the batch"}
+            log(f"train cpu baseline: {train['cpu_baseline']}")
 
     wavefront = None if args.no_wavefront else wavefront_bench(dev, args)
+    voxelize = None if args.no_voxelize else voxelize_bench(dev, args)
 
     # CPU baseline: the oracle's FP32 C loop on a bounded sample
     cpu = None
@@ -313,6 +403,7 @@ def main():
         "cpu_baseline": cpu,
         "train": train,
         "wavefront": wavefront,
+        "voxelize": voxelize,
     }
     print(json.dumps(line), flush=True)
 

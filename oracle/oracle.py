@@ -87,10 +87,15 @@ def voxelize(tris: np.ndarray, res: int, solid: bool = False, pad: bool = False,
         info = voxinfo(mn, mx, res, pad, tris.shape[0])
     table = np.zeros(table_words(res), np.uint32)
     L = lib()
+    prev = L.nlo_omp_max_threads()
     if threads is not None:
         L.nlo_omp_set_threads(int(threads))
-    fn = L.nlo_voxelize_solid if solid else L.nlo_voxelize_surface
-    fn(C.byref(info), _fptr(tris), table.ctypes.data_as(C.POINTER(C.c_uint32)), int(morton))
+    try:
+        fn = L.nlo_voxelize_solid if solid else L.nlo_voxelize_surface
+        fn(C.byref(info), _fptr(tris), table.ctypes.data_as(C.POINTER(C.c_uint32)), int(morton))
+    finally:
+        if threads is not None:  # the thread count is process-wide: put it back
+            L.nlo_omp_set_threads(int(prev))
     return table, info
 
 
