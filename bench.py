@@ -125,6 +125,7 @@ def wavefront_bench(dev, args):
     return res
 
 
+LABEL_RES = 64          # LoD grid of the visibility training labels
 VOX_RES = 256
 VOX_TRIS = 200_000       # the CPU-runnable voxelization workload (blob mesh, 256^3 solid)
 VOX_TRIS_LARGE = 20_000_000
@@ -301,11 +302,17 @@ def main():
         from neural_lod_amd.tcnn import Trainer
         adam = {"otype": "Adam", "learning_rate": 1e-2, "beta1": 0.9, "beta2": 0.99, "epsilon": 1e-15,
                 "l2_reg": 1e-8}
+        from neural_lod_amd.labels import VisibilityLabeler
         nb = TRAIN_BATCH
-        data = []
-        for k in range(4):  # batches resident in HBM, cycled
-            xq = synthetic.visibility_queries(nb, 100 + k)
-            data.append((torch.from_numpy(xq).to(dev), torch.from_numpy(synthetic.sphere_visibility_labels(xq)).to(dev)))
+        # labels: visibility rays from the occupied 64^3 voxels of the 200k-triangle
+        # blob mesh (nearest hit inside the voxel), drawn on the GPU
+        mesh = torch.from_numpy(synthetic.blob_mesh(VOX_TRIS, seed=0)).to(dev)
+        labeler = VisibilityLabeler(mesh, LABEL_RES)
+        data = [labeler.sample(nb, seed=100, first=k * nb) for k in range(4)]  # resident in HBM, cycled
+        label_ms = time_steps(lambda: labeler.sample(nb, seed=101, out=data[0]), 10)
+        data[0] = labeler.sample(nb, seed=100, first=0, out=data[0])
+        positive = float(torch.cat([d[1] for d in data]).mean())
+        log(f"labels: {labeler.n_occupied} occupied voxels, {positive:.3f} positive, {label_ms:.3f} ms per batch")
         train = {}
         for name, prec in (("fp32", FP32), ("fp16", FP16)):
             m = NetworkWithInputEncoding(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"], prec)
@@ -329,7 +336,11 @@ def main():
             del tr, m
         train["speedup_fp16_vs_fp32"] = train["fp32"]["ms_per_step"] / train["fp16"]["ms_per_step"]
         train["batch"] = nb
-        train["labels"] = "analytic sphere occlusion of random rays (synthetic)"
+        train["labels"] = (f"mesh visibility rays: {VOX_TRIS}-triangle blob mesh, origin in one of the "
+                           f"{labeler.n_occupied} occupied {LABEL_RES}^3 voxels, direction on the sphere, "
+                           "1 = nearest hit inside the voxel")
+        train["labels_positive_fraction"] = positive
+        train["label_gen_ms_per_batch"] = label_ms
         if not args.no_cpu:
             # the oracle's training step (FP64 forward/backward, float32 Adam; OpenMP
             # over queries) on 2^14-query batches of the same workload

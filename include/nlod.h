@@ -313,6 +313,35 @@ int nlod_wavefront_run_batches(nlod_wavefront* wavefront, void* stream, uint32_t
                                const float* d_vis_in, const float* d_thr_in, float* d_vis_out, float* d_thr_out,
                                int use_graph);
 
+/* ------------------------------------------ visibility training samples */
+
+/* Rays from the occupied voxels of a res^3 grid over a mesh, labelled with the
+ * renderer's visibility query (rptr/vulkan/raygen.rgen:690-730: nearest hit,
+ * occlusion 1 when the hit point lies inside the voxel).  This produces the
+ * (inputs, targets) batches that neural_lod_learning.cu:588-598 hands to
+ * training_step; it replaces the Vulkan ray-query round trip of
+ * fill_ray_queries_visibility / record_ray_query_results_visibility_optim
+ * (neural_lod_learning_kernels.cuh:364-585) with one GPU kernel.  Sampling:
+ * voxel uniform over the occupied ones, origin uniform inside it, direction
+ * uniform on the sphere; inputs are [position in the grid box (0..1)^3,
+ * (direction + 1) / 2].  Every sample is a pure function of (seed, index). */
+typedef struct nlod_labeler nlod_labeler;
+
+/* Box = cubic mesh box at res^3 (no padding); occupancy = surface
+ * voxelization; per occupied voxel the triangles whose clamped voxel range
+ * contains it.  d_triangles float[n][9] stays caller-owned and must outlive
+ * the labeler.  Synchronizes the stream (set-up on the host). */
+int nlod_labeler_create(void* stream, const float* d_triangles, uint64_t n_triangles, uint32_t res,
+                        nlod_labeler** out);
+int nlod_labeler_destroy(nlod_labeler* labeler);
+/* Grid box and size of the labeler (each output nullable). */
+int nlod_labeler_info(const nlod_labeler* labeler, nlod_voxinfo* grid_out, uint32_t* n_occupied_out,
+                      uint64_t* n_list_entries_out);
+/* Samples first .. first + n - 1: d_inputs float[n][6], d_labels float[n].
+ * Asynchronous. */
+int nlod_labeler_visibility(const nlod_labeler* labeler, void* stream, uint64_t seed, uint64_t first, uint32_t n,
+                            float* d_inputs, float* d_labels);
+
 #ifdef __cplusplus
 }
 #endif

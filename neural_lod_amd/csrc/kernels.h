@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "layout.h"
+#include "nlod.h"
 
 namespace nlod {
 
@@ -36,5 +37,12 @@ void mlp_weight_grad_f32(hipStream_t stream, const NetShape& net, const float* e
                          const float* delta, const float* delta_out, uint32_t N, float* grads, float* workspace,
                          size_t workspace_floats, bool accumulate);
 size_t mlp_weight_grad_workspace_floats(const NetShape& net, uint32_t N);
+
+// ---- voxelize.hip -------------------------------------------------------------
+void voxinfo_from_bbox(const float bmin[3], const float bmax[3], uint32_t G, uint32_t flags, uint64_t n_tri,
+                       struct nlod_voxinfo* info);
+void voxelize(hipStream_t stream, const struct nlod_voxinfo* info, const float* d_tris, uint32_t* d_table,
+              uint32_t flags);
+void mesh_bbox(hipStream_t stream, const float* d_tris, uint64_t n_tri, float bmin[3], float bmax[3]);
 
 }  // namespace nlod

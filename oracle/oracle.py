@@ -48,6 +48,9 @@ def lib() -> C.CDLL:
         L.nlo_voxinfo_from_bbox.argtypes = [fp, fp, C.c_uint32, C.c_float, C.c_uint64, C.POINTER(VoxInfo)]
         L.nlo_voxelize_surface.argtypes = [C.POINTER(VoxInfo), fp, u32p, C.c_int]
         L.nlo_voxelize_solid.argtypes = [C.POINTER(VoxInfo), fp, u32p, C.c_int]
+        L.nlo_visibility_samples.restype = C.c_int64
+        L.nlo_visibility_samples.argtypes = [fp, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, C.c_int,
+                                             fp, fp]
         L.nlo_omp_max_threads.restype = C.c_int
         L.nlo_omp_set_threads.argtypes = [C.c_int]
         _lib = L
@@ -103,3 +106,15 @@ def unpack_bits(table: np.ndarray, res: int) -> np.ndarray:
     """Occupancy as bool[z][y][x] from the MSB-first bit table."""
     bits = np.unpackbits(table.astype(">u4").view(np.uint8))[: res ** 3]
     return bits.reshape(res, res, res).astype(bool)
+
+
+def visibility_samples(tris: np.ndarray, res: int, seed: int, first: int, n: int,
+                       brute_force: bool = False) -> tuple[np.ndarray, np.ndarray, int]:
+    """(inputs [n, 6], labels [n, 1], occupied voxel count) of the visibility
+    sample stream; brute_force tests every triangle instead of the voxel's list."""
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    inputs = np.zeros((n, 6), np.float32)
+    labels = np.zeros((n, 1), np.float32)
+    n_occ = lib().nlo_visibility_samples(_fptr(tris), tris.shape[0], res, int(seed) & (2 ** 64 - 1), first, n,
+                                         int(brute_force), _fptr(inputs), _fptr(labels))
+    return inputs, labels, int(n_occ)
