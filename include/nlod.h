@@ -313,34 +313,51 @@ int nlod_wavefront_run_batches(nlod_wavefront* wavefront, void* stream, uint32_t
                                const float* d_vis_in, const float* d_thr_in, float* d_vis_out, float* d_thr_out,
                                int use_graph);
 
-/* ------------------------------------------ visibility training samples */
+/* ------------------------------------------ training samples from a mesh */
 
-/* Rays from the occupied voxels of a res^3 grid over a mesh, labelled with the
- * renderer's visibility query (rptr/vulkan/raygen.rgen:690-730: nearest hit,
- * occlusion 1 when the hit point lies inside the voxel).  This produces the
- * (inputs, targets) batches that neural_lod_learning.cu:588-598 hands to
- * training_step; it replaces the Vulkan ray-query round trip of
- * fill_ray_queries_visibility / record_ray_query_results_visibility_optim
- * (neural_lod_learning_kernels.cuh:364-585) with one GPU kernel.  Sampling:
- * voxel uniform over the occupied ones, origin uniform inside it, direction
- * uniform on the sphere; inputs are [position in the grid box (0..1)^3,
- * (direction + 1) / 2].  Every sample is a pure function of (seed, index). */
+/* Training batches for the two LoD networks, drawn on the GPU from a triangle
+ * mesh.  They replace the Vulkan ray-query round trip that produces the
+ * (inputs, targets) handed to training_step (neural_lod_learning.cu:588-598,
+ * 686-694): fill_ray_queries_visibility / _throughput, the ray queries of
+ * rptr/vulkan/raygen.rgen:505-730 and record_ray_query_results_* /
+ * copy_valid_results_throughput_optim (neural_lod_learning_kernels.cuh:
+ * 100-330, 364-585).
+ *
+ * The LoD pyramid: level l is the (res >> l)^3 grid over the cubic mesh box
+ * (no padding); level 0 is occupied where the surface voxelization is, level
+ * l where any of its 2^l-block of level-0 voxels is.  Every sample is a pure
+ * function of (seed, index): voxel and level uniform, directions uniform on
+ * the sphere (a simplification of the reference's dilated-box sampler). */
 typedef struct nlod_labeler nlod_labeler;
 
-/* Box = cubic mesh box at res^3 (no padding); occupancy = surface
- * voxelization; per occupied voxel the triangles whose clamped voxel range
- * contains it.  d_triangles float[n][9] stays caller-owned and must outlive
- * the labeler.  Synchronizes the stream (set-up on the host). */
+/* Builds the pyramid and, per occupied voxel of each level, the triangles
+ * whose clamped voxel range contains it.  res must be divisible by
+ * 2^(n_lods - 1); n_lods in [1, 8].  d_triangles float[n][9] stays
+ * caller-owned and must outlive the labeler.  Synchronizes the stream (the
+ * set-up runs on the host). */
 int nlod_labeler_create(void* stream, const float* d_triangles, uint64_t n_triangles, uint32_t res,
-                        nlod_labeler** out);
+                        uint32_t n_lods, nlod_labeler** out);
 int nlod_labeler_destroy(nlod_labeler* labeler);
-/* Grid box and size of the labeler (each output nullable). */
-int nlod_labeler_info(const nlod_labeler* labeler, nlod_voxinfo* grid_out, uint32_t* n_occupied_out,
-                      uint64_t* n_list_entries_out);
-/* Samples first .. first + n - 1: d_inputs float[n][6], d_labels float[n].
- * Asynchronous. */
+/* Grid box, occupied voxels and list entries of one level (outputs nullable). */
+int nlod_labeler_info(const nlod_labeler* labeler, uint32_t lod, nlod_voxinfo* grid_out,
+                      uint32_t* n_occupied_out, uint64_t* n_list_entries_out);
+/* Visibility samples first .. first + n - 1 at level 0 (raygen.rgen:690-730:
+ * origin uniform in an occupied voxel, nearest hit, label 1 when it lies in
+ * the voxel): d_inputs float[n][6] = [position in the box (0..1)^3,
+ * (direction + 1) / 2], d_labels float[n].  Asynchronous. */
 int nlod_labeler_visibility(const nlod_labeler* labeler, void* stream, uint64_t seed, uint64_t first, uint32_t n,
                             float* d_inputs, float* d_labels);
+/* Throughput samples (raygen.rgen:505-688): level, voxel, outgoing wo and
+ * light direction wi per sample; spp paths enter the voxel along -wo, add
+ * next-event estimates along wi at up to max_depth hits inside the voxel and
+ * continue by cosine-weighted bounces off two-sided Lambertian triangles
+ * whose albedo is drawn from albedo_seed.  d_inputs float[n][10] = [voxel
+ * centre in the box, (wo + 1) / 2, (wi + 1) / 2, level], d_targets float[n][3]
+ * = mean RGB throughput of the valid paths (first ray hits inside the voxel),
+ * d_valid float[n] = 1 when any path was valid.  Asynchronous. */
+int nlod_labeler_throughput(const nlod_labeler* labeler, void* stream, uint64_t seed, uint64_t first, uint32_t n,
+                            uint32_t spp, uint32_t max_depth, uint64_t albedo_seed, float* d_inputs,
+                            float* d_targets, float* d_valid);
 
 #ifdef __cplusplus
 }

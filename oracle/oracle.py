@@ -51,6 +51,10 @@ def lib() -> C.CDLL:
         L.nlo_visibility_samples.restype = C.c_int64
         L.nlo_visibility_samples.argtypes = [fp, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, C.c_int,
                                              fp, fp]
+        L.nlo_throughput_samples.restype = C.c_int
+        L.nlo_throughput_samples.argtypes = [fp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64,
+                                             C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, fp, fp, fp,
+                                             u32p]
         L.nlo_omp_max_threads.restype = C.c_int
         L.nlo_omp_set_threads.argtypes = [C.c_int]
         _lib = L
@@ -118,3 +122,21 @@ def visibility_samples(tris: np.ndarray, res: int, seed: int, first: int, n: int
     n_occ = lib().nlo_visibility_samples(_fptr(tris), tris.shape[0], res, int(seed) & (2 ** 64 - 1), first, n,
                                          int(brute_force), _fptr(inputs), _fptr(labels))
     return inputs, labels, int(n_occ)
+
+
+def throughput_samples(tris: np.ndarray, res: int, n_lods: int, seed: int, first: int, n: int, spp: int = 64,
+                       max_depth: int = 3, albedo_seed: int = 1, brute_force: bool = False):
+    """(inputs [n, 10], targets [n, 3], valid [n], occupied voxels per level) of
+    the throughput sample stream."""
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    inputs = np.zeros((n, 10), np.float32)
+    targets = np.zeros((n, 3), np.float32)
+    valid = np.zeros(n, np.float32)
+    n_occ = np.zeros(8, np.uint32)
+    ok = lib().nlo_throughput_samples(_fptr(tris), tris.shape[0], res, n_lods, int(seed) & (2 ** 64 - 1), first, n,
+                                      spp, max_depth, int(albedo_seed) & (2 ** 64 - 1), int(brute_force),
+                                      _fptr(inputs), _fptr(targets), _fptr(valid),
+                                      n_occ.ctypes.data_as(C.POINTER(C.c_uint32)))
+    if not ok:
+        raise ValueError("empty level or bad n_lods")
+    return inputs, targets, valid, [int(v) for v in n_occ[:n_lods]]

@@ -57,3 +57,31 @@ def test_invalid_arguments(cuda):
         VisibilityLabeler(tris, 0)
     with pytest.raises(NlodError):
         VisibilityLabeler(tris[:0], 16)
+
+
+@pytest.mark.parametrize("n_lods", [1, 3])
+def test_throughput_matches_oracle(oracle, cuda, n_lods):
+    from neural_lod_amd.labels import Labeler
+    tris = S.blob_mesh(30_000, seed=1)
+    print(f"case throughput blob30k res=32 n_lods={n_lods}", flush=True)
+    lb = Labeler(torch.from_numpy(tris).cuda(), 32, n_lods)
+    x, t, v = lb.throughput(1024, seed=21, first=777, spp=16, max_depth=3, albedo_seed=5)
+    torch.cuda.synchronize()
+    ex, et, ev, n_occ = oracle.throughput_samples(tris, 32, n_lods, seed=21, first=777, n=1024, spp=16, max_depth=3,
+                                                  albedo_seed=5)
+    assert [lv["n_occupied"] for lv in lb.levels] == n_occ
+    assert np.array_equal(x.cpu().numpy(), ex)
+    assert np.array_equal(v.cpu().numpy(), ev)
+    assert np.array_equal(t.cpu().numpy(), et), f"{int((t.cpu().numpy() != et).any(axis=1).sum())} targets differ"
+    assert 0 < ev.mean() < 1 and et.max() > 0
+
+
+def test_throughput_invalid_arguments(cuda):
+    from neural_lod_amd._lib import NlodError
+    from neural_lod_amd.labels import Labeler
+    tris = torch.from_numpy(S.blob_mesh(1000, seed=4)).cuda()
+    with pytest.raises(NlodError, match="divisible"):
+        Labeler(tris, 30, 3)
+    lb = Labeler(tris, 16, 2)
+    with pytest.raises(NlodError):
+        lb.throughput(8, seed=1, spp=0)
