@@ -131,6 +131,69 @@ VOX_TRIS = 200_000       # the CPU-runnable voxelization workload (blob mesh, 25
 VOX_TRIS_LARGE = 20_000_000
 
 
+THR_LODS = 5
+THR_SPP = 64
+
+
+def throughput_train_bench(dev, args):
+    """Prefiltered-throughput network (hash grid + SH4(wo) + SH4(wi) + LoD
+    scalar, MLP 64 x 5 ReLU, 3 exp outputs), RelativeL2, batch 2^18 of valid
+    Monte-Carlo targets (64 paths each) of the 200k-triangle blob mesh over a
+    5-level pyramid (64^3 .. 4^3): FP16-MFMA and FP32 plain training steps."""
+    import torch
+    from neural_lod_amd import configs, synthetic
+    from neural_lod_amd.labels import Labeler
+    from neural_lod_amd.tcnn import FP16, FP32, NetworkWithInputEncoding, Trainer
+    cfg = configs.THROUGHPUT
+    nb = TRAIN_BATCH
+    mesh = torch.from_numpy(synthetic.blob_mesh(VOX_TRIS, seed=0)).to(dev)
+    lb = Labeler(mesh, LABEL_RES, THR_LODS)
+    gen = 1 << 18
+    xs, ts, made, first = [], [], 0, 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while made < 2 * nb:
+        x, t, v = lb.throughput(gen, seed=200, first=first, spp=THR_SPP, max_depth=3)
+        keep = v > 0
+        xs.append(x[keep])
+        ts.append(t[keep])
+        made += int(keep.sum())
+        first += gen
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    x_all, t_all = torch.cat(xs)[:2 * nb].contiguous(), torch.cat(ts)[:2 * nb].contiguous()
+    data = [(x_all[:nb], t_all[:nb]), (x_all[nb:], t_all[nb:])]
+    out = {"batch": nb, "spp": THR_SPP, "lods": THR_LODS,
+           "targets": f"Monte-Carlo voxel-local throughput, {THR_SPP} paths, 200k-triangle blob mesh, "
+                      f"{LABEL_RES}^3 .. {LABEL_RES >> (THR_LODS - 1)}^3 pyramid, valid samples only",
+           "target_gen_ms_per_1k_samples": gen_s * 1e3 / (first / 1024), "valid_fraction": made / first}
+    # lr 1e-3: at 1e-2 the exp outputs fall to 0, where RelativeL2 has no gradient
+    adam = {"otype": "Adam", "learning_rate": 1e-3, "beta1": 0.9, "beta2": 0.99, "epsilon": 1e-15, "l2_reg": 1e-8}
+    out["optimizer"] = adam
+    for name, prec in (("fp32", FP32), ("fp16", FP16)):
+        m = NetworkWithInputEncoding(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"], prec)
+        tr = Trainer(m, adam, {"otype": "RelativeL2"}, seed=1337, graphs=True)
+        for k in range(max(args.warmup, 2)):
+            tr.training_step(*data[k % 2])
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for k in range(args.train_steps):
+            tr.training_step(*data[k % 2])
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.train_steps
+        st = tr.status()
+        out[name] = {"ms_per_step": ms, "steps_per_s": 1e3 / ms, "final_loss": st["loss"]}
+        log(f"throughput train {name}: {ms:.4f} ms/step, loss {st['loss']:.5f}")
+        del tr, m
+    out["speedup_fp16_vs_fp32"] = out["fp32"]["ms_per_step"] / out["fp16"]["ms_per_step"]
+    del lb, mesh
+    torch.cuda.empty_cache()
+    return out
+
+
 def voxelize_bench(dev, args):
     """Solid and surface voxelization of the seeded 200k-triangle blob mesh at
     256^3 (bit-exact check against the CPU restatement, which is also timed as
@@ -361,6 +424,7 @@ def main():
                                                "scaled to the batch"}
             log(f"train cpu baseline: {train['cpu_baseline']}")
 
+    throughput = None if args.no_train else throughput_train_bench(dev, args)
     wavefront = None if args.no_wavefront else wavefront_bench(dev, args)
     voxelize = None if args.no_voxelize else voxelize_bench(dev, args)
 
@@ -413,6 +477,7 @@ def main():
                      "mlp_tflops": FLOP_PER_QUERY * N_QUERIES / ((fp16["ms"] - enc_ms) * 1e-3) / 1e12},
         "cpu_baseline": cpu,
         "train": train,
+        "throughput_train": throughput,
         "wavefront": wavefront,
         "voxelize": voxelize,
     }

@@ -293,3 +293,33 @@ def test_fp16_loss_curve_mesh_labels(cuda):
     l32, l16 = curves[0][-50:].mean(), curves[1][-50:].mean()
     assert l32 < 0.8 * curves[0][:10].mean(), "the task must be learnable"
     assert abs(l16 - l32) <= 0.02 * l32, f"FP16 {l16:.5f} vs FP32 {l32:.5f}"
+
+
+def test_throughput_loss_curve_mc_targets(cuda):
+    """Throughput network (hash grid + 2 x SH4 + LoD scalar, MLP 64x5, exp
+    outputs) on Monte-Carlo throughput targets of a blob mesh over a 3-level
+    LoD pyramid (valid samples only), RelativeL2: mean loss of the last 50 of
+    300 steps of the FP16-MFMA form within 2% of the FP32 plain form."""
+    from neural_lod_amd.labels import Labeler
+    cfg = configs.THROUGHPUT
+    lb = Labeler(torch.from_numpy(synthetic.blob_mesh(50_000, seed=0)).cuda(), 32, 3)
+    x, t, v = lb.throughput(1 << 16, seed=3, spp=64, max_depth=3)
+    keep = v > 0
+    x, t = x[keep].contiguous(), t[keep].contiguous()
+    nb = 1 << 12
+    assert x.shape[0] >= 8 * nb
+    batches = [(x[(k % 8) * nb:(k % 8 + 1) * nb], t[(k % 8) * nb:(k % 8 + 1) * nb]) for k in range(300)]
+    curves = {}
+    for prec in (0, 1):
+        # lr 1e-3: at 1e-2 the exp outputs fall to 0, where RelativeL2 has no gradient
+        adam = {"otype": "Adam", "learning_rate": 1e-3, "beta1": 0.9, "beta2": 0.99, "epsilon": 1e-15}
+        tr = _trainer(cfg, prec, optimizer=adam, loss={"otype": "RelativeL2"}, seed=1337)
+        losses = []
+        for xb, tb in batches:
+            tr.training_step(xb, tb)
+            losses.append(tr.loss())
+        curves[prec] = np.array(losses)
+        print(prec, curves[prec][::30])
+    l32, l16 = curves[0][-50:].mean(), curves[1][-50:].mean()
+    assert l32 < 0.2 * curves[0][0], "the task must be learnable"
+    assert abs(l16 - l32) <= 0.02 * l32, f"FP16 {l16:.5f} vs FP32 {l32:.5f}"
