@@ -968,8 +968,9 @@ bool tile_geometry(uint32_t G, uint64_t n_tri, bool force, bool accumulate, Tile
     g.tiles_y = (uint32_t)tiles_y;
     g.wpp = (uint32_t)wpp;
     g.n_tiles = (uint32_t)(tiles_y * tiles_y);
-    // at most ~1024 count / scatter blocks, each a multiple of the staging size
-    const uint64_t per = std::max<uint64_t>(2048, (n_tri + 1023) / 1024);
+    // at most ~1024 count / scatter blocks (a row of counts per tile), each a
+    // multiple of the staging size; small meshes still get one block per 256
+    const uint64_t per = std::max<uint64_t>(kVoxThreads, (n_tri + 1023) / 1024);
     g.tris_per_block = (uint32_t)((per + kVoxThreads - 1) / kVoxThreads * kVoxThreads);
     // list entries per workgroup: about 16 chunks per CU for ~2 entries per triangle
     const uint64_t chunk = 2 * n_tri / (16ull * (uint64_t)cu_count());
