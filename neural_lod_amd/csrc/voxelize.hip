@@ -974,7 +974,7 @@ bool tile_geometry(uint32_t G, uint64_t n_tri, bool force, bool accumulate, Tile
     g.tris_per_block = (uint32_t)((per + kVoxThreads - 1) / kVoxThreads * kVoxThreads);
     // list entries per workgroup: about 16 chunks per CU for ~2 entries per triangle
     const uint64_t chunk = 2 * n_tri / (16ull * (uint64_t)cu_count());
-    g.chunk = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(chunk, 512), 1u << 16);
+    g.chunk = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(chunk, kVoxThreads), 1u << 16);
     g.aligned = aligned ? 1 : 0;
     g.accumulate = accumulate ? 1 : 0;
     return true;
