@@ -359,6 +359,33 @@ int nlod_labeler_throughput(const nlod_labeler* labeler, void* stream, uint64_t 
                             uint32_t spp, uint32_t max_depth, uint64_t albedo_seed, float* d_inputs,
                             float* d_targets, float* d_valid);
 
+/* ------------------------------------------ LoD occupancy pyramid, ray marching */
+
+/* A stack of occupancy bit tables in the voxelizer's layout: level 0 is a
+ * copy of d_table (a G^3 voxelization described by info), level l + 1 the OR
+ * of 2x2x2 blocks of level l.  Replaces the per-LoD nanovdb grids built from
+ * the downsampled point cloud (neural_lod_learning_init.cu:201-295).
+ * G must be divisible by 2^(n_lods - 1); n_lods in [1, 10].  Asynchronous. */
+typedef struct nlod_lod_grid nlod_lod_grid;
+int nlod_lod_grid_create(void* stream, const nlod_voxinfo* info, const uint32_t* d_table, uint32_t n_lods,
+                         nlod_lod_grid** out);
+int nlod_lod_grid_destroy(nlod_lod_grid* grid);
+/* Device bit table and box of one level (outputs nullable; the table stays
+ * owned by the grid). */
+int nlod_lod_grid_level(const nlod_lod_grid* grid, uint32_t lod, const uint32_t** d_table_out,
+                        nlod_voxinfo* info_out);
+/* First occupied voxel of level `lod` along each ray o + t d, t >= t_min
+ * (d_t_min nullable: 0), skipping voxel d_prev_voxel[i] (nullable; linear
+ * index x + y R + z R^2 or 0xffffffff): d_voxel_out = its linear index or
+ * 0xffffffff when the ray leaves the grid, d_t_near_out / d_t_far_out the
+ * ray's entry and exit parameters of that voxel's box (-1 when none).
+ * Replaces first_active_voxel_intersection (neural_lod_learning_common.cuh:
+ * 784-832) as raymarch_impl uses it (wavefront_neural_pt.cuh:89-141).
+ * Origins and directions float[n][3], world space.  Asynchronous. */
+int nlod_lod_grid_march(const nlod_lod_grid* grid, void* stream, uint32_t lod, uint32_t n, const float* d_origins,
+                        const float* d_dirs, const float* d_t_min, const uint32_t* d_prev_voxel,
+                        uint32_t* d_voxel_out, float* d_t_near_out, float* d_t_far_out);
+
 #ifdef __cplusplus
 }
 #endif

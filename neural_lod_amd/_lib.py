@@ -101,6 +101,10 @@ SIGNATURES: dict[str, list] = {
     "nlod_labeler_info": [_vp, _u32, C.POINTER(VoxInfo), C.POINTER(_u32), C.POINTER(_u64)],
     "nlod_labeler_visibility": [_vp, _vp, _u64, _u64, _u32, _vp, _vp],
     "nlod_labeler_throughput": [_vp, _vp, _u64, _u64, _u32, _u32, _u32, _u64, _vp, _vp, _vp],
+    "nlod_lod_grid_create": [_vp, C.POINTER(VoxInfo), _vp, _u32, C.POINTER(_vp)],
+    "nlod_lod_grid_destroy": [_vp],
+    "nlod_lod_grid_level": [_vp, _u32, C.POINTER(_vp), C.POINTER(VoxInfo)],
+    "nlod_lod_grid_march": [_vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
 }
 
 

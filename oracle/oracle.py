@@ -55,6 +55,8 @@ def lib() -> C.CDLL:
         L.nlo_throughput_samples.argtypes = [fp, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64,
                                              C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_int, fp, fp, fp,
                                              u32p]
+        L.nlo_downsample.argtypes = [u32p, C.c_uint32, u32p]
+        L.nlo_march.argtypes = [u32p, C.c_uint32, fp, fp, C.c_uint32, fp, fp, C.c_void_p, C.c_void_p, u32p, fp, fp]
         L.nlo_omp_max_threads.restype = C.c_int
         L.nlo_omp_set_threads.argtypes = [C.c_int]
         _lib = L
@@ -140,3 +142,32 @@ def throughput_samples(tris: np.ndarray, res: int, n_lods: int, seed: int, first
     if not ok:
         raise ValueError("empty level or bad n_lods")
     return inputs, targets, valid, [int(v) for v in n_occ[:n_lods]]
+
+
+def downsample(table: np.ndarray, res: int) -> np.ndarray:
+    """Bit table of the (res/2)^3 level: OR of 2x2x2 blocks."""
+    child = np.ascontiguousarray(table, dtype=np.uint32)
+    parent = np.zeros(table_words(res // 2), np.uint32)
+    u32 = C.POINTER(C.c_uint32)
+    lib().nlo_downsample(child.ctypes.data_as(u32), res, parent.ctypes.data_as(u32))
+    return parent
+
+
+def march(table: np.ndarray, res: int, bmin, unit, origins: np.ndarray, dirs: np.ndarray, t_min=None, prev=None):
+    """(voxel uint32 [n] with 0xffffffff for none, t_near [n], t_far [n])."""
+    bits = np.ascontiguousarray(table, dtype=np.uint32)
+    o = np.ascontiguousarray(origins, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32)
+    n = o.shape[0]
+    bm = np.ascontiguousarray(bmin, np.float32)
+    un = np.ascontiguousarray(unit, np.float32)
+    vox = np.zeros(n, np.uint32)
+    tn = np.zeros(n, np.float32)
+    tf = np.zeros(n, np.float32)
+    tm = None if t_min is None else np.ascontiguousarray(t_min, np.float32)
+    pv = None if prev is None else np.ascontiguousarray(prev, np.uint32)
+    u32 = C.POINTER(C.c_uint32)
+    lib().nlo_march(bits.ctypes.data_as(u32), res, _fptr(bm), _fptr(un), n, _fptr(o), _fptr(d),
+                    None if tm is None else tm.ctypes.data, None if pv is None else pv.ctypes.data,
+                    vox.ctypes.data_as(u32), _fptr(tn), _fptr(tf))
+    return vox, tn, tf

@@ -1,0 +1,55 @@
+"""GPU LoD pyramid and first-active-voxel march (libnlod nlod_lod_grid_*)
+against the CPU restatement: identical tables, voxels and t values."""
+import numpy as np
+import pytest
+import torch
+
+from neural_lod_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("res,n_lods", [(64, 4), (96, 3)])
+def test_pyramid_and_march(oracle, cuda, res, n_lods):
+    from neural_lod_amd import voxelizer as V
+    from neural_lod_amd.lod_grid import LodGrid
+    tris = S.blob_mesh(30_000, seed=2)
+    print(f"case pyramid res={res} n_lods={n_lods}", flush=True)
+    table, info = V.voxelize(torch.from_numpy(tris).cuda(), res)
+    grid = LodGrid(table, info, n_lods)
+    level = table.cpu().numpy().view(np.uint32)
+    rng = np.random.default_rng(5)
+    n = 20000
+    for lod in range(n_lods):
+        r = res >> lod
+        got = grid.table(lod).cpu().numpy().view(np.uint32)
+        if lod > 0:
+            level = oracle.downsample(level, r * 2)
+        assert np.array_equal(got, level), f"level {lod} table differs"
+        li = grid.levels[lod]
+        bmin, unit = np.array(li.bbox_min), np.array(li.unit)
+        o = (bmin + rng.uniform(-0.3, r + 0.3, size=(n, 3)) * unit).astype(np.float32)
+        d = rng.normal(size=(n, 3))
+        d[rng.random(n) < 0.05, 0] = 0.0  # axis-parallel rays
+        d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+        t_min = rng.uniform(0.0, 0.2, n).astype(np.float32)
+        prev = np.where(rng.random(n) < 0.3, rng.integers(0, r ** 3, n), 0xFFFFFFFF).astype(np.uint32)
+        v, tn, tf = grid.march(lod, torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda(),
+                               torch.from_numpy(t_min).cuda(), torch.from_numpy(prev.view(np.int32)).cuda())
+        ev, etn, etf = oracle.march(level, r, bmin, unit, o, d, t_min, prev)
+        exp = np.where(ev == 0xFFFFFFFF, -1, ev.astype(np.int64))
+        assert np.array_equal(v.cpu().numpy(), exp), f"lod {lod}: voxels differ"
+        assert np.array_equal(tn.cpu().numpy(), etn) and np.array_equal(tf.cpu().numpy(), etf)
+        assert (exp >= 0).mean() > 0.05
+
+
+def test_invalid_arguments(cuda):
+    from neural_lod_amd import voxelizer as V
+    from neural_lod_amd._lib import NlodError
+    from neural_lod_amd.lod_grid import LodGrid
+    table, info = V.voxelize(torch.from_numpy(S.blob_mesh(100, seed=1)).cuda(), 24)
+    with pytest.raises(NlodError, match="divisible"):
+        LodGrid(table, info, 5)
+    g = LodGrid(table, info, 2)
+    with pytest.raises(NlodError):
+        g.march(2, torch.zeros(4, 3, device="cuda"), torch.ones(4, 3, device="cuda"))
