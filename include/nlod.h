@@ -386,6 +386,37 @@ int nlod_lod_grid_march(const nlod_lod_grid* grid, void* stream, uint32_t lod, u
                         const float* d_dirs, const float* d_t_min, const uint32_t* d_prev_voxel,
                         uint32_t* d_voxel_out, float* d_t_near_out, float* d_t_far_out);
 
+/* ------------------------------------------ per-voxel visibility thresholds */
+
+/* Scores of NeuralLodLearning::ThresholdOptimMetrics, same order
+ * (neural_lod_learning.h:105-118; formulas neural_lod_learning_common.cuh:
+ * 836-927). */
+enum nlod_threshold_metric {
+    NLOD_METRIC_PRECISION = 0,
+    NLOD_METRIC_RECALL = 1,
+    NLOD_METRIC_FSCORE = 2,
+    NLOD_METRIC_FSCORE_MACRO = 3,
+    NLOD_METRIC_FSCORE_WEIGHTED = 4,
+    NLOD_METRIC_OPTIMAL_FSCORE_WEIGHTED = 5,
+    NLOD_METRIC_MCC = 6,
+    NLOD_METRIC_COHEN_KAPPA = 7,
+    NLOD_METRIC_ACCURACY = 8,
+    NLOD_METRIC_BALANCED_ACCURACY = 9,
+    NLOD_METRIC_FOWLKES_MALLOWS = 10,
+    NLOD_METRIC_YOUDEN_J = 11
+};
+
+/* Per voxel v: the threshold t / (T - 1) maximising `metric` of the
+ * predictions p > t / (T - 1) against labels (1.0 = occluded) over samples
+ * [v S, v S + S) of d_labels / d_probas, ties resolved to the middle of the
+ * plateau that follows the first maximum.  Replaces compute_confusion_matrices,
+ * compute_scores and compute_optimal_thresholds (neural_lod_learning.cu:
+ * 122-266).  d_thresholds_out float[n_voxels]; d_scores_out (nullable)
+ * float[n_voxels][T].  T in [2, 1024].  Asynchronous. */
+int nlod_optimal_thresholds(void* stream, uint32_t n_voxels, uint32_t samples_per_voxel, const float* d_labels,
+                            const float* d_probas, uint32_t n_thresholds, int metric, float beta,
+                            float* d_thresholds_out, float* d_scores_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,6 +105,7 @@ SIGNATURES: dict[str, list] = {
     "nlod_lod_grid_destroy": [_vp],
     "nlod_lod_grid_level": [_vp, _u32, C.POINTER(_vp), C.POINTER(VoxInfo)],
     "nlod_lod_grid_march": [_vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "nlod_optimal_thresholds": [_vp, _u32, _u32, _vp, _vp, _u32, _i32, C.c_float, _vp, _vp],
 }
 
 

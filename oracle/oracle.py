@@ -57,6 +57,7 @@ def lib() -> C.CDLL:
                                              u32p]
         L.nlo_downsample.argtypes = [u32p, C.c_uint32, u32p]
         L.nlo_march.argtypes = [u32p, C.c_uint32, fp, fp, C.c_uint32, fp, fp, C.c_void_p, C.c_void_p, u32p, fp, fp]
+        L.nlo_optimal_thresholds.argtypes = [C.c_uint32, C.c_uint32, fp, fp, C.c_uint32, C.c_int, C.c_float, fp, fp]
         L.nlo_omp_max_threads.restype = C.c_int
         L.nlo_omp_set_threads.argtypes = [C.c_int]
         _lib = L
@@ -171,3 +172,16 @@ def march(table: np.ndarray, res: int, bmin, unit, origins: np.ndarray, dirs: np
                     None if tm is None else tm.ctypes.data, None if pv is None else pv.ctypes.data,
                     vox.ctypes.data_as(u32), _fptr(tn), _fptr(tf))
     return vox, tn, tf
+
+
+def optimal_thresholds(labels: np.ndarray, probas: np.ndarray, n_voxels: int, n_thresholds: int = 101,
+                       metric: int = 2, beta: float = 1.0):
+    """(thresholds [V], scores [V][T]) of the per-voxel threshold optimisation."""
+    lab = np.ascontiguousarray(labels, np.float32).ravel()
+    pro = np.ascontiguousarray(probas, np.float32).ravel()
+    S = lab.size // n_voxels
+    thr = np.zeros(n_voxels, np.float32)
+    sc = np.zeros((n_voxels, n_thresholds), np.float32)
+    lib().nlo_optimal_thresholds(n_voxels, S, _fptr(lab), _fptr(pro), n_thresholds, metric, beta, _fptr(thr),
+                                 _fptr(sc))
+    return thr, sc
