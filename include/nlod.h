@@ -251,6 +251,19 @@ int nlod_trainer_serialize(nlod_trainer* trainer, int with_optimizer_state, void
                            size_t* needed);
 int nlod_trainer_deserialize(nlod_trainer* trainer, const void* buf, size_t len);
 
+/* The weight file of the reference (Trainer::serialize without optimizer
+ * state, trainer.h:267-278, as write_weights_to_disk writes it,
+ * neural_lod_learning_ui.cu:916-924): JSON text
+ * {"n_params":N,"params_binary":{"bytes":[...],"subtype":null},"params_type":"float"}
+ * holding the inference parameters in tcnn's order.  buf NULL: *needed =
+ * size including the terminating NUL.  Synchronizes the trainer's stream. */
+int nlod_trainer_weights_json(nlod_trainer* trainer, char* buf, size_t buf_len, size_t* needed);
+/* Loads such a file (load_weights_from_disk, neural_lod_learning_ui.cu:
+ * 935-947; Trainer::deserialize, trainer.h:280-307): params_type "float" or
+ * "__half", bytes in the {"bytes": [...]} text form; an "optimizer" entry is
+ * ignored.  Sets the master and inference parameters.  Synchronizes. */
+int nlod_trainer_load_weights_json(nlod_trainer* trainer, void* stream, const char* json, size_t len);
+
 /* Loss evaluation alone (Loss::evaluate with stride == dims):
  * values (nullable) and loss_scale * dL/dpred per element, n_total = n_elements.
  * loss_type: 0 L2 (losses/l2.h), 1 RelativeL2 (losses/relative_l2.h:39-76),

@@ -86,6 +86,8 @@ SIGNATURES: dict[str, list] = {
     "nlod_trainer_set_params": [_vp, _vp, _vp],
     "nlod_trainer_serialize": [_vp, _i32, _vp, C.c_size_t, C.POINTER(C.c_size_t)],
     "nlod_trainer_deserialize": [_vp, _vp, C.c_size_t],
+    "nlod_trainer_weights_json": [_vp, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)],
+    "nlod_trainer_load_weights_json": [_vp, _vp, C.c_char_p, C.c_size_t],
     "nlod_loss_evaluate": [_vp, _i32, _u64, C.c_float, _vp, _vp, _vp, _vp],
     "nlod_adam_step": [_vp, _vp, _u64, _u64, C.c_float, _u32, _vp, _vp, _vp, _vp, _vp],
     "nlod_wavefront_create": [_vp, _vp, C.c_float, C.POINTER(_vp)],

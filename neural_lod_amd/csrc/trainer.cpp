@@ -338,6 +338,93 @@ struct Trainer {
         bind(stream);
         NLOD_HIP(hipStreamSynchronize(stream));
     }
+
+    // tcnn's weight file (Trainer::serialize without the optimizer,
+    // trainer.h:267-278, written as JSON text by write_weights_to_disk,
+    // neural_lod_learning_ui.cu:916-924): nlohmann's text form of a binary
+    // value is {"bytes": [...], "subtype": null} (gpu_memory_json.h:36-71);
+    // keys sorted as nlohmann's object map sorts them.
+    std::string weights_json() {
+        NLOD_HIP(hipStreamSynchronize(stream));
+        std::vector<uint8_t> bytes(sizeof(float) * n);
+        NLOD_HIP(hipMemcpy(bytes.data(), opt.ema ? ema.p() : params.p(), bytes.size(), hipMemcpyDeviceToHost));
+        std::string out;
+        out.reserve(bytes.size() * 4 + 128);
+        out += "{\"n_params\":" + std::to_string(n) + ",\"params_binary\":{\"bytes\":[";
+        char num[4];
+        for (size_t i = 0; i < bytes.size(); ++i) {
+            if (i) out += ',';
+            const unsigned v = bytes[i];
+            int k = 0;
+            if (v >= 100) num[k++] = (char)('0' + v / 100);
+            if (v >= 10) num[k++] = (char)('0' + (v / 10) % 10);
+            num[k++] = (char)('0' + v % 10);
+            out.append(num, (size_t)k);
+        }
+        out += "],\"subtype\":null},\"params_type\":\"float\"}";
+        return out;
+    }
+
+    // Trainer::deserialize (trainer.h:280-307): params_type "float" or "__half";
+    // params_binary as the {"bytes": [...]} object of the text form.  Other
+    // keys (an "optimizer" state) are ignored.
+    void load_weights_json(hipStream_t caller, const char* js, size_t len) {
+        const std::string text(js, len);
+        auto value_of = [&](const char* key) -> size_t {  // index just past the key's ':'
+            const std::string k = std::string("\"") + key + "\"";
+            const size_t at = text.find(k);
+            if (at == std::string::npos) return std::string::npos;
+            const size_t colon = text.find(':', at + k.size());
+            return colon == std::string::npos ? colon : colon + 1;
+        };
+        std::string type = "float";
+        const size_t tv = value_of("params_type");
+        if (tv != std::string::npos) {
+            const size_t q0 = text.find('"', tv), q1 = q0 == std::string::npos ? q0 : text.find('"', q0 + 1);
+            if (q1 == std::string::npos) fail(NLOD_ERR_INVALID_ARGUMENT, "weights JSON: bad params_type");
+            type = text.substr(q0 + 1, q1 - q0 - 1);
+        }
+        if (type != "float" && type != "__half")
+            fail(NLOD_ERR_INVALID_ARGUMENT, "weights JSON: params_type must be float or __half");
+        const size_t bv = value_of("params_binary");
+        if (bv == std::string::npos) fail(NLOD_ERR_INVALID_ARGUMENT, "weights JSON: no params_binary");
+        const size_t bytes_key = text.find("\"bytes\"", bv);
+        const size_t open = bytes_key == std::string::npos ? bytes_key : text.find('[', bytes_key);
+        if (open == std::string::npos) fail(NLOD_ERR_INVALID_ARGUMENT, "weights JSON: params_binary has no bytes");
+        std::vector<uint8_t> bytes;
+        bytes.reserve(sizeof(float) * n);
+        size_t i = open + 1;
+        for (;;) {
+            while (i < len && (text[i] == ' ' || text[i] == '\n' || text[i] == '\t' || text[i] == '\r' || text[i] == ','))
+                ++i;
+            if (i >= len) fail(NLOD_ERR_INVALID_ARGUMENT, "weights JSON: unterminated bytes");
+            if (text[i] == ']') break;
+            unsigned v = 0;
+            int digits = 0;
+            while (i < len && text[i] >= '0' && text[i] <= '9' && digits < 4) v = v * 10 + (unsigned)(text[i++] - '0'), ++digits;
+            if (digits == 0 || v > 255) fail(NLOD_ERR_INVALID_ARGUMENT, "weights JSON: bytes must be 0..255");
+            bytes.push_back((uint8_t)v);
+        }
+        const size_t elem = type == "float" ? 4 : 2;
+        if (bytes.size() != elem * n)
+            fail(NLOD_ERR_INVALID_ARGUMENT, "weights JSON holds " + std::to_string(bytes.size() / elem) +
+                                                " parameters, the model has " + std::to_string(n));
+        std::vector<float> host(n);
+        if (elem == 4) {
+            std::memcpy(host.data(), bytes.data(), bytes.size());
+        } else {
+            for (size_t k = 0; k < n; ++k) {
+                _Float16 h;
+                std::memcpy(&h, bytes.data() + 2 * k, 2);
+                host[k] = (float)h;
+            }
+        }
+        DevBuf tmp;
+        tmp.ensure(sizeof(float) * n);
+        NLOD_HIP(hipMemcpy(tmp.p, host.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        set_params(caller, tmp.as<float>());
+        NLOD_HIP(hipStreamSynchronize(caller));  // tmp is released on return
+    }
 };
 
 }  // namespace nlod
@@ -449,6 +536,25 @@ int nlod_trainer_deserialize(nlod_trainer* tr, const void* buf, size_t len) {
         NLOD_CHECK(tr && buf, "null argument");
         tr->t->drop_graphs();
         tr->t->deserialize(static_cast<const uint8_t*>(buf), len);
+    });
+}
+
+int nlod_trainer_weights_json(nlod_trainer* tr, char* buf, size_t buf_len, size_t* needed) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(tr, "null trainer");
+        const std::string js = tr->t->weights_json();
+        if (needed) *needed = js.size() + 1;
+        if (!buf) return;
+        NLOD_CHECK(buf_len >= js.size() + 1, "weights JSON buffer too small");
+        std::memcpy(buf, js.c_str(), js.size() + 1);
+    });
+}
+
+int nlod_trainer_load_weights_json(nlod_trainer* tr, void* stream, const char* json, size_t len) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(tr && json, "null argument");
+        tr->t->drop_graphs();
+        tr->t->load_weights_json(nlod::as_stream(stream), json, len);
     });
 }
 

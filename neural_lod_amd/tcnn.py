@@ -259,6 +259,19 @@ class Trainer:
     def deserialize(self, blob: bytes) -> None:
         _lib.call("nlod_trainer_deserialize", self._h, C.c_char_p(blob), len(blob))
 
+    def weights_json(self) -> str:
+        """The reference's weight file text (tcnn Trainer::serialize as JSON)."""
+        need = C.c_size_t()
+        _lib.call("nlod_trainer_weights_json", self._h, None, 0, C.byref(need))
+        buf = C.create_string_buffer(need.value)
+        _lib.call("nlod_trainer_weights_json", self._h, buf, need.value, C.byref(need))
+        return buf.value.decode()
+
+    def load_weights_json(self, text: str | bytes) -> None:
+        """Load a reference weight file (params_type float or __half)."""
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        _lib.call("nlod_trainer_load_weights_json", self._h, _stream(torch.device("cuda")), data, len(data))
+
 
 def loss_evaluate(loss_type: int, pred: torch.Tensor, target: torch.Tensor, loss_scale: float = 1.0):
     """Per-element loss values and scaled gradients (tcnn Loss::evaluate)."""

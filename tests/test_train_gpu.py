@@ -323,3 +323,33 @@ def test_throughput_loss_curve_mc_targets(cuda):
     l32, l16 = curves[0][-50:].mean(), curves[1][-50:].mean()
     assert l32 < 0.2 * curves[0][0], "the task must be learnable"
     assert abs(l16 - l32) <= 0.02 * l32, f"FP16 {l16:.5f} vs FP32 {l32:.5f}"
+
+
+def test_weights_json_roundtrip(cuda):
+    """The reference's weight file: nlohmann's JSON text of Trainer::serialize
+    (sorted keys, binary as {"bytes": [...], "subtype": null}); float and
+    __half payloads load back; a wrong size is rejected."""
+    import json
+    from neural_lod_amd._lib import NlodError
+    cfg = SMALL_VIS
+    tr = _trainer(cfg, 0, seed=3)
+    for x, t in _batches(512, 3, 40):
+        tr.training_step(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda())
+    text = tr.weights_json()
+    assert text.startswith('{"n_params":') and text.endswith('"params_type":"float"}')
+    d = json.loads(text)
+    assert d["params_type"] == "float" and d["params_binary"]["subtype"] is None
+    w = np.array(d["params_binary"]["bytes"], np.uint8).view(np.float32)
+    assert d["n_params"] == w.size
+    assert np.array_equal(w, tr.inference_params().cpu().numpy())
+    tr2 = _trainer(cfg, 0, seed=99)
+    tr2.load_weights_json(text)
+    assert torch.equal(tr2.params(), tr.inference_params())
+    h = w.astype(np.float16)
+    d16 = {"n_params": int(w.size), "optimizer": {"otype": "Adam"},
+           "params_binary": {"bytes": list(h.tobytes()), "subtype": None}, "params_type": "__half"}
+    tr2.load_weights_json(json.dumps(d16, indent=1))
+    assert np.array_equal(tr2.params().cpu().numpy(), h.astype(np.float32))
+    d16["params_binary"]["bytes"] = d16["params_binary"]["bytes"][:-2]
+    with pytest.raises(NlodError, match="parameters"):
+        tr2.load_weights_json(json.dumps(d16))
