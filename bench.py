@@ -131,6 +131,7 @@ VOX_TRIS = 200_000       # the CPU-runnable voxelization workload (blob mesh, 25
 VOX_TRIS_LARGE = 20_000_000
 
 
+THR_RES = 256     # finest level of the throughput-target pyramid (256^3 .. 16^3)
 THR_LODS = 5
 THR_SPP = 64
 
@@ -139,7 +140,7 @@ def throughput_train_bench(dev, args):
     """Prefiltered-throughput network (hash grid + SH4(wo) + SH4(wi) + LoD
     scalar, MLP 64 x 5 ReLU, 3 exp outputs), RelativeL2, batch 2^18 of valid
     Monte-Carlo targets (64 paths each) of the 200k-triangle blob mesh over a
-    5-level pyramid (64^3 .. 4^3): FP16-MFMA and FP32 plain training steps."""
+    5-level pyramid (256^3 .. 16^3): FP16-MFMA and FP32 plain training steps."""
     import torch
     from neural_lod_amd import configs, synthetic
     from neural_lod_amd.labels import Labeler
@@ -147,7 +148,7 @@ def throughput_train_bench(dev, args):
     cfg = configs.THROUGHPUT
     nb = TRAIN_BATCH
     mesh = torch.from_numpy(synthetic.blob_mesh(VOX_TRIS, seed=0)).to(dev)
-    lb = Labeler(mesh, LABEL_RES, THR_LODS)
+    lb = Labeler(mesh, THR_RES, THR_LODS)
     gen = 1 << 18
     xs, ts, made, first = [], [], 0, 0
     torch.cuda.synchronize()
@@ -165,7 +166,7 @@ def throughput_train_bench(dev, args):
     data = [(x_all[:nb], t_all[:nb]), (x_all[nb:], t_all[nb:])]
     out = {"batch": nb, "spp": THR_SPP, "lods": THR_LODS,
            "targets": f"Monte-Carlo voxel-local throughput, {THR_SPP} paths, 200k-triangle blob mesh, "
-                      f"{LABEL_RES}^3 .. {LABEL_RES >> (THR_LODS - 1)}^3 pyramid, valid samples only",
+                      f"{THR_RES}^3 .. {THR_RES >> (THR_LODS - 1)}^3 pyramid, valid samples only",
            "target_gen_ms_per_1k_samples": gen_s * 1e3 / (first / 1024), "valid_fraction": made / first}
     # lr 1e-3: at 1e-2 the exp outputs fall to 0, where RelativeL2 has no gradient
     adam = {"otype": "Adam", "learning_rate": 1e-3, "beta1": 0.9, "beta2": 0.99, "epsilon": 1e-15, "l2_reg": 1e-8}
