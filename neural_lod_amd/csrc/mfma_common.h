@@ -212,16 +212,37 @@ __device__ __forceinline__ void grid_level_h16(const GridLevels& lv, uint32_t le
         load_entry<F, half_t>(tab + (size_t)idx * F, v);
         return;
     }
+    // p = (dy, dz); corners c = 2p (x0) and 2p + 1 (x0 + 1).  All block loads are
+    // issued before any is used, so a lane keeps its up to 8 gathers in flight
+    // together instead of waiting for each in turn.
+    uint32_t i0[4], i1[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const uint32_t y = cell[1] + (p & 1), z = cell[2] + ((p >> 1) & 1);
+        i0[p] = grid_index(lv.grid_type, hs, res, cell[0], y, z);
+        i1[p] = grid_index(lv.grid_type, hs, res, cell[0] + 1, y, z);
+    }
+    uint4 blk0[4], blk1[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+        blk0[p] = *reinterpret_cast<const uint4*>(tab + (size_t)(i0[p] / PER_BLOCK) * PER_BLOCK * F);
+    bool shared[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        // the x0 + 1 corner shares x0's block 3 times in 4 (F = 2); otherwise it needs
+        // its own block.  The load is issued either way, without a branch (a branch
+        // makes the compiler wait for the loads before it): lanes whose corner is
+        // shared read the level's first block, one line that the whole wave shares
+        // and that stays in the vector L1, so it costs no L2 request per lane.
+        shared[p] = i0[p] / PER_BLOCK == i1[p] / PER_BLOCK;
+        const size_t b1 = shared[p] ? 0 : (size_t)(i1[p] / PER_BLOCK) * PER_BLOCK * F;
+        blk1[p] = *reinterpret_cast<const uint4*>(tab + b1);
+    }
     float vals[8][F];
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {  // p = (dy, dz); corners c = 2p (x0) and 2p + 1 (x0 + 1)
-        const uint32_t y = cell[1] + (p & 1), z = cell[2] + ((p >> 1) & 1);
-        const uint32_t i0 = grid_index(lv.grid_type, hs, res, cell[0], y, z);
-        const uint32_t i1 = grid_index(lv.grid_type, hs, res, cell[0] + 1, y, z);
-        const uint4 blk = *reinterpret_cast<const uint4*>(tab + (size_t)(i0 / PER_BLOCK) * PER_BLOCK * F);
-        entry_of_block<F>(blk, i0 % PER_BLOCK, vals[2 * p]);
-        if (i0 / PER_BLOCK == i1 / PER_BLOCK) entry_of_block<F>(blk, i1 % PER_BLOCK, vals[2 * p + 1]);
-        else load_entry<F, half_t>(tab + (size_t)i1 * F, vals[2 * p + 1]);
+    for (int p = 0; p < 4; ++p) {
+        entry_of_block<F>(blk0[p], i0[p] % PER_BLOCK, vals[2 * p]);
+        entry_of_block<F>(shared[p] ? blk0[p] : blk1[p], i1[p] % PER_BLOCK, vals[2 * p + 1]);
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
@@ -230,6 +251,84 @@ __device__ __forceinline__ void grid_level_h16(const GridLevels& lv, uint32_t le
         for (int d = 0; d < 3; ++d) w *= ((c >> d) & 1) ? fr[d] : 1.0f - fr[d];
 #pragma unroll
         for (int f = 0; f < F; ++f) v[f] += w * vals[c][f];
+    }
+}
+
+// One level of a grid for one query from an FP16 table, on two lanes: an even
+// lane and the odd lane after it (s = lane parity) share the query.  Lane s
+// gathers the x0 + s corner of each of the 4 (y, z) rows, so within one load
+// instruction the two corners of an x-edge come from adjacent lanes, and since
+// they almost always lie in one cache line the pair costs one L2 request
+// rather than two.  The lanes then swap their gathers (DPP quad_perm
+// [1, 0, 3, 2]) and lane s interpolates the features of parity s: v[k] is
+// feature 2k + s (F = 1: both lanes compute feature 0).  Per feature, the same
+// arithmetic and order as grid_level.  Both lanes of a pair must be active.
+template <int F>
+__device__ __forceinline__ void grid_level_h16_pair(const GridLevels& lv, uint32_t level, float x0, float x1,
+                                                    float x2, const half_t* __restrict__ table, uint32_t s,
+                                                    float* v) {
+#pragma clang fp contract(off)
+    constexpr int WORDS = F == 1 ? 1 : F / 2;  // 32-bit words per entry (F = 1: the half in the low bits)
+    const float scale = lv.scale[level];
+    const uint32_t res = lv.res[level];
+    const uint32_t off = lv.offset[level];
+    const uint32_t hs = lv.offset[level + 1] - off;
+    const half_t* tab = table + (size_t)off * F;
+    float fr[3];
+    uint32_t cell[3];
+    grid_pos(x0, scale, lv.interp, fr[0], cell[0]);
+    grid_pos(x1, scale, lv.interp, fr[1], cell[1]);
+    grid_pos(x2, scale, lv.interp, fr[2], cell[2]);
+    auto load_words = [&](uint32_t idx, uint32_t* w) {
+        const half_t* e = tab + (size_t)idx * F;
+        if constexpr (F == 1) {
+            w[0] = __builtin_bit_cast(uint16_t, e[0]);
+        } else if constexpr (F == 2) {
+            w[0] = *reinterpret_cast<const uint32_t*>(e);
+        } else if constexpr (F == 4) {
+            const uint2 t = *reinterpret_cast<const uint2*>(e);
+            w[0] = t.x;
+            w[1] = t.y;
+        } else {
+            const uint4 t = *reinterpret_cast<const uint4*>(e);
+            w[0] = t.x;
+            w[1] = t.y;
+            w[2] = t.z;
+            w[3] = t.w;
+        }
+    };
+    // feature 2k + s (F = 1: feature 0) of an entry held as words
+    auto feature = [&](const uint32_t* w, int k) -> float {
+        const uint32_t bits = F == 1 ? w[0] : (w[k] >> (16 * s)) & 0xffffu;
+        return (float)__builtin_bit_cast(half_t, (uint16_t)bits);
+    };
+#pragma unroll
+    for (int k = 0; k < WORDS; ++k) v[k] = 0.0f;
+    if (lv.interp == INTERP_NEAREST) {
+        uint32_t w[WORDS];
+        load_words(grid_index(lv.grid_type, hs, res, cell[0], cell[1], cell[2]), w);
+#pragma unroll
+        for (int k = 0; k < WORDS; ++k) v[k] = feature(w, k);
+        return;
+    }
+    uint32_t mine[4][WORDS], other[4][WORDS];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+        load_words(grid_index(lv.grid_type, hs, res, cell[0] + s, cell[1] + (p & 1), cell[2] + ((p >> 1) & 1)),
+                   mine[p]);
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int k = 0; k < WORDS; ++k)
+            other[p][k] = (uint32_t)__builtin_amdgcn_mov_dpp((int)mine[p][k], 0xB1, 0xF, 0xF, false);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        float w = 1.0f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) w *= ((c >> d) & 1) ? fr[d] : 1.0f - fr[d];
+        const uint32_t* e = ((uint32_t)(c & 1) == s) ? mine[c >> 1] : other[c >> 1];
+#pragma unroll
+        for (int k = 0; k < WORDS; ++k) v[k] += w * feature(e, k);
     }
 }
 

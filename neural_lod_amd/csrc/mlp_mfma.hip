@@ -1,7 +1,9 @@
 // FP16-MFMA form of the encode + fully fused MLP (gfx950).
 //
 // Two launches per batch:
-//  1. Encoding.  One thread per (query, grid level), blockIdx.y = level, so
+//  1. Encoding.  Two lanes per (query, grid level), one for each x-side of the
+//     cell (grid_level_h16_pair: the two corners of an x-edge then cost one L2
+//     request), blockIdx.y = level, so
 //     the workgroups resident at any time all gather from one level's table:
 //     that level (2 MiB as FP16 at 2^19 entries x 2 features) stays in each
 //     XCD's 4 MiB L2 while 2^20 queries stream through it (the reference's
@@ -60,33 +62,40 @@ __global__ __launch_bounds__(kBlock) void encode_kernel(uint32_t N, uint32_t N16
                                                         const EncProgram* __restrict__ prog,
                                                         const TT* __restrict__ table, const float* __restrict__ in,
                                                         half_t* __restrict__ feat) {
-    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
-    if (q >= N16) return;
+    static_assert(std::is_same<TT, half_t>::value, "the encoding pass gathers from the FP16 table copy");
     const EncProgram& P = *prog;
     const uint32_t kp2 = P.k_padded / 2, kp8 = P.k_padded / 8;
-    const bool valid = q < N;
-    const float* x = in + (size_t)(valid ? q : 0) * P.n_input_dims;
     if (blockIdx.y < map.n) {
+        // grid rows: two lanes per query (grid_level_h16_pair); N16 is a multiple
+        // of 16, so both lanes of a pair return together
+        const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+        const uint32_t q = t >> 1, lane_s = t & 1u;
+        if (q >= N16) return;
+        const bool valid = q < N;
+        const float* x = in + (size_t)(valid ? q : 0) * P.n_input_dims;
         const DevSegment& s = P.segs[map.seg[blockIdx.y]];
         const uint32_t level = map.level[blockIdx.y];
         const float* xs = x + s.in_offset;
-        float v[F];
-        if constexpr (std::is_same<TT, half_t>::value)
-            grid_level_h16<F>(s.lv, level, xs[0], xs[1], xs[2], table + s.param_offset, v);
-        else
-            grid_level<F, TT>(s.lv, level, xs[0], xs[1], xs[2], table + s.param_offset, v);
+        constexpr int VF = F == 1 ? 1 : F / 2;
+        float v[VF];
+        grid_level_h16_pair<F>(s.lv, level, xs[0], xs[1], xs[2], table + s.param_offset, lane_s, v);
         const uint32_t f0 = s.out_offset + level * F;  // multiple of F
-        half_t h[F];
-#pragma unroll
-        for (int f = 0; f < F; ++f) h[f] = valid ? (half_t)v[f] : (half_t)0.0f;
         if constexpr (F == 1) {
-            feat[feat_pair_offset(q >> 4, f0 >> 1, kp2, q & 15) + (f0 & 1)] = h[0];
+            if (lane_s == 0)
+                feat[feat_pair_offset(q >> 4, f0 >> 1, kp2, q & 15) + (f0 & 1)] = valid ? (half_t)v[0] : (half_t)0.0f;
         } else {
+            // lane s writes element s of each of the level's feature pairs
 #pragma unroll
-            for (int j = 0; j < F / 2; ++j) *pair_ptr(feat, q, f0 + 2 * j, kp2) = pack2(h[2 * j], h[2 * j + 1]);
+            for (int k = 0; k < VF; ++k)
+                feat[feat_pair_offset(q >> 4, (f0 >> 1) + k, kp2, q & 15) + lane_s] =
+                    valid ? (half_t)v[k] : (half_t)0.0f;
         }
         return;
     }
+    const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= N16) return;
+    const bool valid = q < N;
+    const float* x = in + (size_t)(valid ? q : 0) * P.n_input_dims;
     // non-grid rows: SH, identity, padding (grid pad rows and the K tail are 0)
     for (uint32_t g8 = 0; g8 < kp8; ++g8) {
         const uint32_t mask = P.grid_rows[g8];
@@ -259,7 +268,7 @@ void mfma_encode(hipStream_t stream, const Layout& L, const void* d_program, con
     if (N == 0) return;
     const LevelMap map = make_level_map(L);
     const uint32_t N16 = div_round_up(N, 16) * 16;
-    const dim3 grid(div_round_up(N16, kBlock), map.n + 1);
+    const dim3 grid(div_round_up(2 * N16, kBlock), map.n + 1);  // grid rows take two lanes per query
     dispatch_fused(L, [&](auto Wc, auto Fc, auto Dc) {
         constexpr int F = decltype(Fc)::value, D = decltype(Dc)::value;
         hipLaunchKernelGGL((encode_kernel<F, D, half_t>), grid, dim3(kBlock), 0, stream, N, N16, map,
