@@ -65,6 +65,28 @@ __device__ __forceinline__ float act_fwd(int a, float v) {
     }
 }
 
+// act_fwd over N values, the switch taken once for all of them: the activation
+// is uniform over the wave, and per element the compiler branches around every
+// value, which multiplies the instruction count of an MFMA layer several times.
+template <int N>
+__device__ __forceinline__ void act_fwd_n(int a, float* v) {
+    switch (a) {
+        case ACT_RELU:
+#pragma unroll
+            for (int i = 0; i < N; ++i) v[i] = v[i] > 0.0f ? v[i] : 0.0f;
+            break;
+        case ACT_SIGMOID:
+#pragma unroll
+            for (int i = 0; i < N; ++i) v[i] = 1.0f / (1.0f + __expf(-v[i]));
+            break;
+        case ACT_EXP:
+#pragma unroll
+            for (int i = 0; i < N; ++i) v[i] = __expf(v[i]);
+            break;
+        default: break;
+    }
+}
+
 // derivative from the post-activation value y (common_device.h:193-244)
 __device__ __forceinline__ float act_bwd(int a, float y) {
     switch (a) {
@@ -72,6 +94,29 @@ __device__ __forceinline__ float act_bwd(int a, float y) {
         case ACT_SIGMOID: return y * (1.0f - y);
         case ACT_EXP: return y;
         default: return 1.0f;
+    }
+}
+
+// act_bwd over N values (d[i] = act'(y[i])), the switch taken once
+template <int N>
+__device__ __forceinline__ void act_bwd_n(int a, const float* y, float* d) {
+    switch (a) {
+        case ACT_RELU:
+#pragma unroll
+            for (int i = 0; i < N; ++i) d[i] = y[i] > 0.0f ? 1.0f : 0.0f;
+            break;
+        case ACT_SIGMOID:
+#pragma unroll
+            for (int i = 0; i < N; ++i) d[i] = y[i] * (1.0f - y[i]);
+            break;
+        case ACT_EXP:
+#pragma unroll
+            for (int i = 0; i < N; ++i) d[i] = y[i];
+            break;
+        default:
+#pragma unroll
+            for (int i = 0; i < N; ++i) d[i] = 1.0f;
+            break;
     }
 }
 

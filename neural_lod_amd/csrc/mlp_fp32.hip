@@ -129,10 +129,12 @@ __global__ __launch_bounds__(kThreads) void mlp_fwd_kernel(uint32_t N, MlpArgs a
         for (uint32_t layer = 0;; ++layer) {
             // activation -> LDS tile (and the saved activations for backward)
 #pragma unroll
+            for (int i = 0; i < UB; ++i) act_fwd_n<QB>(a.act, acc[i]);
+#pragma unroll
             for (int i = 0; i < UB; ++i)
 #pragma unroll
                 for (int j = 0; j < QB; ++j) {
-                    const float h = act_fwd(a.act, acc[i][j]);
+                    const float h = acc[i][j];
                     cur[(u0 + i) * T + q0 + j] = h;
                     if (hidden && tq0 + q0 + j < N) hidden[((size_t)layer * W + u0 + i) * N + tq0 + q0 + j] = h;
                 }
@@ -209,14 +211,21 @@ __global__ __launch_bounds__(kThreads) void mlp_bwd_delta_kernel(uint32_t N, Mlp
         for (int layer = (int)L - 1; layer >= 0; --layer) {
             // through the hidden activation: delta = acc * act'(h_layer)
 #pragma unroll
-            for (int i = 0; i < UB; ++i)
+            for (int i = 0; i < UB; ++i) {
+                float hv[QB], dv[QB];
 #pragma unroll
                 for (int j = 0; j < QB; ++j) {
                     const uint32_t q = tq0 + q0 + j;
-                    const float hv = q < N ? hidden[((size_t)layer * W + u0 + i) * N + q] : 0.0f;
-                    acc[i][j] *= act_bwd(a.act, hv);
+                    hv[j] = q < N ? hidden[((size_t)layer * W + u0 + i) * N + q] : 0.0f;
+                }
+                act_bwd_n<QB>(a.act, hv, dv);
+#pragma unroll
+                for (int j = 0; j < QB; ++j) {
+                    const uint32_t q = tq0 + q0 + j;
+                    acc[i][j] *= dv[j];
                     if (q < N) delta[((size_t)layer * W + u0 + i) * N + q] = acc[i][j];
                 }
+            }
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < UB; ++i)

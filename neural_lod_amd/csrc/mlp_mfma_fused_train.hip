@@ -216,10 +216,14 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
         for (int l = L - 1; l >= 0; --l) {
 #pragma unroll
             for (int c = 0; c < KC; ++c) {
+                float y[8], d[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) y[i] = (float)h[l][c][i];
+                act_bwd_n<8>(a.act, y, d);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    dl[c][i] = (half_t)(acc[2 * c][i] * act_bwd(a.act, (float)h[l][c][i]));
-                    dl[c][i + 4] = (half_t)(acc[2 * c + 1][i] * act_bwd(a.act, (float)h[l][c][i + 4]));
+                    dl[c][i] = (half_t)(acc[2 * c][i] * d[i]);
+                    dl[c][i + 4] = (half_t)(acc[2 * c + 1][i] * d[i + 4]);
                 }
             }
             stage_units<KC>(bufA, dl, col);

@@ -201,13 +201,19 @@ __global__ __launch_bounds__(kBlock) void mlp_bwd_kernel(TrainArgs a, const half
             // delta_layer = G * act'(H_layer), stored unit-major, packed as next B fragments
 #pragma unroll
             for (int c = 0; c < KC; ++c) {
+                float y[8], dv[8];  // units 32c + 4g + i (i < 4) and 16 above them
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t u0 = 32 * c + 4 * g + i;
+                    y[i] = (float)Hl[unit_major(tile, W, u0, n)];
+                    y[i + 4] = (float)Hl[unit_major(tile, W, u0 + 16, n)];
+                }
+                act_bwd_n<8>(a.act, y, dv);
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const uint32_t u0 = 32 * c + 4 * g + i, u1 = u0 + 16;
-                    const float h0 = (float)Hl[unit_major(tile, W, u0, n)];
-                    const float h1 = (float)Hl[unit_major(tile, W, u1, n)];
-                    const half_t d0 = (half_t)(acc[2 * c][i] * act_bwd(a.act, h0));
-                    const half_t d1 = (half_t)(acc[2 * c + 1][i] * act_bwd(a.act, h1));
+                    const half_t d0 = (half_t)(acc[2 * c][i] * dv[i]);
+                    const half_t d1 = (half_t)(acc[2 * c + 1][i] * dv[i + 4]);
                     b[c][i] = d0;
                     b[c][i + 4] = d1;
                     Dl[unit_major(tile, W, u0, n)] = d0;
