@@ -38,13 +38,21 @@ COMMON_FLAGS = [
 ]
 
 
+# MFMA kernels: accumulators in VGPRs (gfx950 MFMAs read and write either file),
+# so the activation code reads them without v_accvgpr_read copies
+FILE_FLAGS = {
+    name: ["-mllvm", "-amdgpu-mfma-vgpr-form"]
+    for name in ("mlp_mfma.hip", "mlp_mfma_train.hip", "mlp_mfma_fused_train.hip", "wavefront.hip")
+}
+
+
 def _sources() -> list[Path]:
     return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
 
 
 def _compile(src: Path, verbose: bool) -> Path:
     obj = BUILD / (src.name + ".o")
-    cmd = [HIPCC, *COMMON_FLAGS, "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *COMMON_FLAGS, *FILE_FLAGS.get(src.name, []), "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd[1:1] = ["-x", "hip"]
     res = subprocess.run(cmd, capture_output=True, text=True)

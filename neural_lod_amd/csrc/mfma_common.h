@@ -422,9 +422,20 @@ __device__ __forceinline__ floatx4 mfma16(const half8& a, const half8& b, const 
 
 // Activated accumulator tiles 2c and 2c+1 as the B fragment of chunk c of the next layer.
 __device__ __forceinline__ half8 pack_act(const floatx4& lo, const floatx4& hi, int act) {
+    half8 r;
+    if (act == ACT_RELU) {
+        // ReLU on the rounded halves, two per instruction: rounding keeps the
+        // sign and max(NaN, 0) = 0, so only the sign of a zero can differ from
+        // rounding max(v, 0)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            r[i] = (half_t)lo[i];
+            r[i + 4] = (half_t)hi[i];
+        }
+        return __builtin_elementwise_max(r, half8{0, 0, 0, 0, 0, 0, 0, 0});
+    }
     float v[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
     act_fwd_n<8>(act, v);
-    half8 r;
 #pragma unroll
     for (int i = 0; i < 8; ++i) r[i] = (half_t)v[i];
     return r;

@@ -127,38 +127,73 @@ struct MlpArgs16 {
 
 // TT tiles of 16 queries per wave iteration: each weight fragment read from
 // LDS feeds TT MFMAs (LDS bandwidth for the weights bounds the single-tile loop).
-template <int W, int TT>
+// ACT: the hidden activation when fixed at compile time (ReLU), or -1 to read it
+// from the arguments.  KC0 > 0: the input width in 32-feature chunks, fixed at
+// compile time, and each wave loads the feature fragments of its next group
+// while it computes the current one (the loads are then in flight for a whole
+// group's MFMAs); KC0 = 0: width from the arguments, loaded as used.
+template <int W, int TT, int ACT, int KC0>
 __global__ __launch_bounds__(kBlock) void mlp_fwd_kernel(MlpArgs16 a, const half_t* __restrict__ packed,
                                                          const half_t* __restrict__ feat, float* __restrict__ out) {
     constexpr int MT = W / 16, KC = W / 32;
+    constexpr int PF = KC0 > 0 ? KC0 : 1;
+    const int act = ACT >= 0 ? ACT : a.act;
     extern __shared__ float4 smem4[];
     half_t* sw = reinterpret_cast<half_t*>(smem4);
     for (uint32_t k = threadIdx.x; k < a.packed_halves / 8; k += kBlock)
         reinterpret_cast<float4*>(sw)[k] = reinterpret_cast<const float4*>(packed)[k];
     __syncthreads();
     const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
-    const uint32_t kc0 = a.k_padded / 32, kp2 = a.k_padded / 2;
+    const uint32_t kc0 = KC0 > 0 ? (uint32_t)KC0 : a.k_padded / 32, kp2 = a.k_padded / 2;
     const uint32_t n_tiles = (a.N + 15) / 16;
     const uint32_t n_groups = (n_tiles + TT - 1) / TT;
     const uint32_t n_waves = gridDim.x * (kBlock / 64);
-    for (uint32_t grp = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); grp < n_groups; grp += n_waves) {
+    const uint32_t grp0 = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    half8 nb[TT][PF];  // KC0 > 0: fragments of the wave's next group
+    auto load_group = [&](uint32_t gi) {
+#pragma unroll
+        for (int t = 0; t < TT; ++t) {
+            const uint32_t tile = min(gi * TT + t, n_tiles - 1);  // a clamped duplicate is not stored
+#pragma unroll
+            for (int c = 0; c < PF; ++c) nb[t][c] = load_feat_frag(feat, tile, kp2, c, g, n);
+        }
+    };
+    if (KC0 > 0 && grp0 < n_groups) load_group(grp0);
+    for (uint32_t grp = grp0; grp < n_groups; grp += n_waves) {
         floatx4 acc[TT][MT];
 #pragma unroll
         for (int t = 0; t < TT; ++t)
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) acc[t][mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-        for (uint32_t c = 0; c < kc0; ++c) {
-            half8 b[TT];
+        if constexpr (KC0 > 0) {
+            half8 b[TT][PF];
 #pragma unroll
-            for (int t = 0; t < TT; ++t) {
-                const uint32_t tile = min(grp * TT + t, n_tiles - 1);  // a clamped duplicate is not stored
-                b[t] = load_feat_frag(feat, tile, kp2, c, g, n);
-            }
+            for (int t = 0; t < TT; ++t)
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                const half8 w = *reinterpret_cast<const half8*>(sw + ((size_t)(mt * kc0 + c) * 64 + lane) * 8);
+                for (int c = 0; c < PF; ++c) b[t][c] = nb[t][c];
+            if (grp + n_waves < n_groups) load_group(grp + n_waves);
 #pragma unroll
-                for (int t = 0; t < TT; ++t) acc[t][mt] = mfma16(w, b[t], acc[t][mt]);
+            for (int c = 0; c < PF; ++c)
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    const half8 w = *reinterpret_cast<const half8*>(sw + ((size_t)(mt * PF + c) * 64 + lane) * 8);
+#pragma unroll
+                    for (int t = 0; t < TT; ++t) acc[t][mt] = mfma16(w, b[t][c], acc[t][mt]);
+                }
+        } else {
+            for (uint32_t c = 0; c < kc0; ++c) {
+                half8 b[TT];
+#pragma unroll
+                for (int t = 0; t < TT; ++t) {
+                    const uint32_t tile = min(grp * TT + t, n_tiles - 1);
+                    b[t] = load_feat_frag(feat, tile, kp2, c, g, n);
+                }
+#pragma unroll
+                for (int mt = 0; mt < MT; ++mt) {
+                    const half8 w = *reinterpret_cast<const half8*>(sw + ((size_t)(mt * kc0 + c) * 64 + lane) * 8);
+#pragma unroll
+                    for (int t = 0; t < TT; ++t) acc[t][mt] = mfma16(w, b[t], acc[t][mt]);
+                }
             }
         }
         const half_t* frag = sw + (size_t)MT * kc0 * 512;
@@ -166,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void mlp_fwd_kernel(MlpArgs16 a, const half
 #pragma unroll
         for (int t = 0; t < TT; ++t)
 #pragma unroll
-            for (int c = 0; c < KC; ++c) hb[t][c] = pack_act(acc[t][2 * c], acc[t][2 * c + 1], a.act);
+            for (int c = 0; c < KC; ++c) hb[t][c] = pack_act(acc[t][2 * c], acc[t][2 * c + 1], act);
         for (uint32_t layer = 1; layer < a.n_hidden_layers; ++layer) {
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
@@ -186,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void mlp_fwd_kernel(MlpArgs16 a, const half
 #pragma unroll
             for (int t = 0; t < TT; ++t)
 #pragma unroll
-                for (int c = 0; c < KC; ++c) hb[t][c] = pack_act(acc[t][2 * c], acc[t][2 * c + 1], a.act);
+                for (int c = 0; c < KC; ++c) hb[t][c] = pack_act(acc[t][2 * c], acc[t][2 * c + 1], act);
         }
         floatx4 o[TT];
 #pragma unroll
@@ -209,6 +244,19 @@ __global__ __launch_bounds__(kBlock) void mlp_fwd_kernel(MlpArgs16 a, const half
                 }
             }
         }
+    }
+}
+
+// ReLU networks with inputs up to 128 features take the prefetching kernels
+template <int W, int TT, typename Launch>
+void launch_fwd(const MlpArgs16& a, Launch&& launch) {
+    if (a.act != ACT_RELU) return launch(mlp_fwd_kernel<W, TT, -1, 0>, TT);
+    switch (a.k_padded / 32) {
+        case 1: return launch(mlp_fwd_kernel<W, TT, ACT_RELU, 1>, TT);
+        case 2: return launch(mlp_fwd_kernel<W, TT, ACT_RELU, 2>, TT);
+        case 3: return launch(mlp_fwd_kernel<W, TT, ACT_RELU, 3>, TT);
+        case 4: return launch(mlp_fwd_kernel<W, TT, ACT_RELU, 4>, TT);
+        default: return launch(mlp_fwd_kernel<W, TT, ACT_RELU, 0>, TT);
     }
 }
 
@@ -299,9 +347,9 @@ void mfma_mlp_forward(hipStream_t stream, const Layout& L, const void* packed, c
                            static_cast<const half_t*>(feat), out);
     };
     switch (L.net.width) {
-        case 32: launch(mlp_fwd_kernel<32, 2>, 2); break;
-        case 64: launch(mlp_fwd_kernel<64, 2>, 2); break;
-        case 128: launch(mlp_fwd_kernel<128, 1>, 1); break;
+        case 32: launch_fwd<32, 2>(a, launch); break;
+        case 64: launch_fwd<64, 2>(a, launch); break;
+        case 128: launch_fwd<128, 1>(a, launch); break;
         default: fail(NLOD_ERR_UNSUPPORTED, "FP16 MFMA kernels support widths 32, 64 and 128");
     }
     NLOD_LAUNCH_CHECK();
