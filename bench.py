@@ -37,8 +37,8 @@ BYTES_PER_QUERY = ENC_BYTES_PER_QUERY + 48 * 2 + 4
 FLOP_PER_QUERY = 2 * (48 * 64 + 3 * 64 * 64 + 64 * 1)
 # HBM bytes per launch of the encoding pass at 2^20 queries, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE passes calibrated by a 256 MiB copy
-# (bench.py --profile-encode; 663.2 MB fetched + 137.2 MB written)
-ENC_TRAFFIC_BYTES = 800.4e6
+# (bench.py --profile-encode; 604.3 MB fetched + 134.8 MB written)
+ENC_TRAFFIC_BYTES = 739.1e6
 
 
 def log(msg):
