@@ -900,21 +900,53 @@ __global__ void init_bbox_kernel(uint32_t* acc) {
     if (threadIdx.x < 3) { acc[threadIdx.x] = 0xffffffffu; acc[3 + threadIdx.x] = 0u; }
 }
 
-__global__ __launch_bounds__(256) void bbox_kernel(const float* __restrict__ tris, uint64_t n_vals, uint32_t* acc) {
-    // n_vals = 9 * n_tri floats; component of value k is k % 3
-    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
-    const uint64_t n_vert = n_vals / 3;
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n_vert;
-         v += (uint64_t)gridDim.x * blockDim.x) {
+// Bounding box of n_vals = 9 n_tri floats (the component of float k is k % 3).
+// With a 16-byte aligned array the block reads tiles of 3 x 256 float4s (3072
+// floats, so a tile starts at component 0), each load instruction one
+// contiguous run across the block: thread t's float4 t + 256 j holds floats
+// whose components are (t + j + e) % 3 for element e, so the thread keeps its
+// minima per phase (j + e) % 3 and maps phases to components at the end.  The
+// rest, or an unaligned array, goes float by float.  NaN coordinates are skipped.
+constexpr int kBboxThreads = 256;
+
+__device__ __forceinline__ void bbox_add(uint32_t* mn, uint32_t* mx, int d, float f) {
+    if (isnan(f)) return;
+    const uint32_t o = f2ord(f);
+    mn[d] = min(mn[d], o);
+    mx[d] = max(mx[d], o);
+}
+
+__global__ __launch_bounds__(kBboxThreads) void bbox_kernel(const float* __restrict__ tris, uint64_t n_vals,
+                                                            bool aligned16, uint32_t* acc) {
+    constexpr uint64_t kTile = 3 * kBboxThreads;  // float4s per tile
+    uint32_t pmn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, pmx[3] = {0u, 0u, 0u};
+    const uint32_t t = threadIdx.x;
+    const uint64_t n_tiles = aligned16 ? n_vals / (4 * kTile) : 0;
+    const float4* t4 = reinterpret_cast<const float4*>(tris);
+    for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        float4 v[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            float f = tris[v * 3 + d];
-            if (isnan(f)) continue;
-            uint32_t o = f2ord(f);
-            mn[d] = min(mn[d], o);
-            mx[d] = max(mx[d], o);
+        for (int j = 0; j < 3; ++j) v[j] = t4[tile * kTile + t + (uint64_t)kBboxThreads * j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            bbox_add(pmn, pmx, (j + 0) % 3, v[j].x);
+            bbox_add(pmn, pmx, (j + 1) % 3, v[j].y);
+            bbox_add(pmn, pmx, (j + 2) % 3, v[j].z);
+            bbox_add(pmn, pmx, (j + 3) % 3, v[j].w);
         }
     }
+    // phase p holds component (t + p) % 3
+    uint32_t mn[3], mx[3];
+    const uint32_t r = t % 3;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const uint32_t p = (uint32_t)(d + 3 - (int)r) % 3;
+        mn[d] = p == 0 ? pmn[0] : p == 1 ? pmn[1] : pmn[2];
+        mx[d] = p == 0 ? pmx[0] : p == 1 ? pmx[1] : pmx[2];
+    }
+    const uint64_t gid = (uint64_t)blockIdx.x * kBboxThreads + t;
+    const uint64_t stride = (uint64_t)gridDim.x * kBboxThreads;
+    for (uint64_t k = n_tiles * 4 * kTile + gid; k < n_vals; k += stride) bbox_add(mn, mx, (int)(k % 3), tris[k]);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         for (int off = 32; off > 0; off >>= 1) {
@@ -922,11 +954,18 @@ __global__ __launch_bounds__(256) void bbox_kernel(const float* __restrict__ tri
             mx[d] = max(mx[d], (uint32_t)__shfl_xor((int)mx[d], off));
         }
     }
-    if ((threadIdx.x & 63) == 0) {
+    __shared__ uint32_t wave_mm[kBboxThreads / 64][6];
+    if ((t & 63) == 0)
         for (int d = 0; d < 3; ++d) {
-            atomicMin(acc + d, mn[d]);
-            atomicMax(acc + 3 + d, mx[d]);
+            wave_mm[t >> 6][d] = mn[d];
+            wave_mm[t >> 6][3 + d] = mx[d];
         }
+    __syncthreads();
+    if (t < 6) {
+        uint32_t m = wave_mm[0][t];
+        for (int w = 1; w < kBboxThreads / 64; ++w) m = t < 3 ? min(m, wave_mm[w][t]) : max(m, wave_mm[w][t]);
+        if (t < 3) atomicMin(acc + t, m);
+        else atomicMax(acc + t, m);
     }
 }
 
@@ -1142,9 +1181,12 @@ void mesh_bbox(hipStream_t stream, const float* d_tris, uint64_t n_tri, float bm
     uint32_t* acc = bbox_accumulator();
     hipLaunchKernelGGL(init_bbox_kernel, dim3(1), dim3(64), 0, stream, acc);
     NLOD_LAUNCH_CHECK();
-    const uint64_t n_vert = n_tri * 3;
-    const uint32_t blocks = (uint32_t)std::min<uint64_t>(div_round_up(n_vert, 256), 2048);
-    hipLaunchKernelGGL(bbox_kernel, dim3(blocks), dim3(256), 0, stream, d_tris, n_tri * 9, acc);
+    const bool aligned16 = (reinterpret_cast<uintptr_t>(d_tris) & 15u) == 0;
+    const uint64_t n_vals = n_tri * 9;
+    const uint64_t per_block = aligned16 ? 12 * kBboxThreads : kBboxThreads;  // floats per block step
+    const uint32_t blocks =
+        (uint32_t)std::min<uint64_t>(div_round_up(n_vals, per_block), (uint64_t)cu_count() * 4);
+    hipLaunchKernelGGL(bbox_kernel, dim3(blocks), dim3(kBboxThreads), 0, stream, d_tris, n_vals, aligned16, acc);
     NLOD_LAUNCH_CHECK();
     uint32_t h[6];
     NLOD_HIP(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, stream));

@@ -157,3 +157,26 @@ def test_invalid_arguments(cuda):
         V.voxelize(tri, 0)
     with pytest.raises(ValueError):
         V.voxelize(tri, 64, path="fast")
+
+
+@pytest.mark.parametrize("n_tri", [1, 2, 3, 4, 5, 7, 1000, 200_001])
+@pytest.mark.parametrize("offset", [0, 1])
+def test_mesh_bbox(cuda, n_tri, offset):
+    """Device bounding box vs numpy: chunked float4 path (16-byte aligned array),
+    float-by-float tail, and a 4-byte offset array (float-by-float throughout);
+    NaN coordinates are skipped."""
+    from neural_lod_amd import voxelizer as V
+    rng = np.random.default_rng(n_tri + offset)
+    tris = rng.normal(size=(n_tri, 9)).astype(np.float32) * 3.0 + 1.0
+    if n_tri > 2:
+        tris[1, 4] = np.nan  # a y coordinate
+        tris[-1, 0] = np.nan  # an x coordinate in the tail
+    flat = torch.zeros(n_tri * 9 + offset, dtype=torch.float32, device="cuda")
+    flat[offset:] = torch.from_numpy(tris.reshape(-1)).cuda()
+    t = flat[offset:].view(n_tri, 9)
+    assert (t.data_ptr() % 16 == 0) == (offset == 0)
+    print(f"case bbox n_tri={n_tri} offset={offset}", flush=True)
+    mn, mx = V.mesh_bbox(t)
+    v = tris.reshape(-1, 3)
+    np.testing.assert_array_equal(mn, np.nanmin(v, axis=0))
+    np.testing.assert_array_equal(mx, np.nanmax(v, axis=0))
