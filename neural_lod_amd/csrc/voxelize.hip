@@ -764,10 +764,21 @@ __global__ __launch_bounds__(kVoxThreads) void tile_scatter_kernel(uint64_t n_tr
     __syncthreads();
     const uint64_t b0 = (uint64_t)blockIdx.x * geo.tris_per_block;
     const uint64_t b1 = min<uint64_t>(n_tri, b0 + geo.tris_per_block);
-    for (uint64_t i = b0 + threadIdx.x; i < b1; i += kVoxThreads) {
-        const uint64_t rt = routes[i];
-        if (rt < kRouteDirect)
-            for_tiles(rt, geo, [&](uint32_t tile) { idx[atomicAdd(&cursor[tile], 1u)] = (uint32_t)i; });
+    // four routes per lane in flight before the first is used
+    constexpr int U = 4;
+    for (uint64_t i0 = b0 + threadIdx.x; i0 < b1; i0 += U * kVoxThreads) {
+        uint64_t rt[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t i = i0 + (uint64_t)u * kVoxThreads;
+            rt[u] = i < b1 ? routes[i] : kRouteNone;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = (uint32_t)(i0 + (uint64_t)u * kVoxThreads);
+            if (rt[u] < kRouteDirect)
+                for_tiles(rt[u], geo, [&](uint32_t tile) { idx[atomicAdd(&cursor[tile], 1u)] = i; });
+        }
     }
 }
 
@@ -1007,9 +1018,11 @@ bool tile_geometry(uint32_t G, uint64_t n_tri, bool force, bool accumulate, Tile
     g.tiles_y = (uint32_t)tiles_y;
     g.wpp = (uint32_t)wpp;
     g.n_tiles = (uint32_t)(tiles_y * tiles_y);
-    // at most ~1024 count / scatter blocks (a row of counts per tile), each a
-    // multiple of the staging size; small meshes still get one block per 256
-    const uint64_t per = std::max<uint64_t>(kVoxThreads, (n_tri + 1023) / 1024);
+    // count / scatter blocks: 1024 to 4096 as the count matrix (one row of block
+    // counts per tile) stays within 8M entries, each a multiple of the staging
+    // size; small meshes still get one block per 256 triangles
+    const uint64_t target = std::min<uint64_t>(4096, std::max<uint64_t>(1024, (8ull << 20) / g.n_tiles));
+    const uint64_t per = std::max<uint64_t>(kVoxThreads, (n_tri + target - 1) / target);
     g.tris_per_block = (uint32_t)((per + kVoxThreads - 1) / kVoxThreads * kVoxThreads);
     // list entries per workgroup: about 16 chunks per CU for ~2 entries per triangle
     const uint64_t chunk = 2 * n_tri / (16ull * (uint64_t)cu_count());
