@@ -467,20 +467,28 @@ __device__ __forceinline__ bool top_left_edge(f2 a, f2 b) {
     return (b.y < a.y) || (b.y == a.y && a.x > b.x);
 }
 
-// cpu_voxelizer.cpp:265-325 for one triangle, restricted to sample rows y in
-// [clip.y0, clip.y1] and z in [clip.z0, clip.z1].  ROWS_ONLY: return the rows
-// the walk would visit instead (empty for triangles that set nothing).
-template <bool ROWS_ONLY, class Sink>
-__device__ Rows solid_triangle(const VoxParams& p, const float* t, Sink* out, const Rows clip) {
-    if (!tri_finite(t)) return {1, 0, 1, 0};
+// Per-triangle part of cpu_voxelizer.cpp:265-325: the triangle relative to the
+// box, its unit normal, the YZ projection turned counter-clockwise, and the
+// sample rows of its projected bounding box (clamped to the table).
+struct SolidTri {
+    f2 a, b, c;
+    f3 n, v0;
+    int y0, y1, z0, z1;
+    uint32_t top_left;  // bit 0: edge (a, b), bit 1: (b, c), bit 2: (c, a)
+    float pad[3];       // 80 bytes: five 16-byte LDS reads
+};
+
+// false: the triangle sets nothing (non-finite, or parallel to the x axis)
+__device__ __forceinline__ bool solid_setup(const VoxParams& p, const float* t, SolidTri& st) {
+    if (!tri_finite(t)) return false;
     const f3 bmin = {p.bmin[0], p.bmin[1], p.bmin[2]};
     f3 v0 = sub3({t[0], t[1], t[2]}, bmin);
     f3 v1 = sub3({t[3], t[4], t[5]}, bmin);
     f3 v2 = sub3({t[6], t[7], t[8]}, bmin);
     f3 e0 = sub3(v1, v0), e1 = sub3(v2, v1);
     f3 n = normalize3(cross3(e0, e1));
-    if ((double)fabsf(n.x) < kFloatError) return {1, 0, 1, 0};
-    if (!(isfinite(n.x) && isfinite(n.y) && isfinite(n.z))) return {1, 0, 1, 0};
+    if ((double)fabsf(n.x) < kFloatError) return false;
+    if (!(isfinite(n.x) && isfinite(n.y) && isfinite(n.z))) return false;
 
     f2 a = {v0.y, v0.z}, b = {v1.y, v1.z}, c = {v2.y, v2.z};
     {
@@ -490,7 +498,7 @@ __device__ Rows solid_triangle(const VoxParams& p, const float* t, Sink* out, co
     }
     const float bmax_x = gmax(a.x, gmax(b.x, c.x)), bmax_y = gmax(a.y, gmax(b.y, c.y));
     const float bmin_x = gmin(a.x, gmin(b.x, c.x)), bmin_y = gmin(a.y, gmin(b.y, c.y));
-    const float uy = p.unit[1], uz = p.unit[2], ux = p.unit[0];
+    const float uy = p.unit[1], uz = p.unit[2];
     const float gmax_y = (float)floor((double)(bmax_x / uy) - 0.5);
     const float gmax_z = (float)floor((double)(bmax_y / uz) - 0.5);
     const float gmin_y = (float)ceil((double)(bmin_x / uy) - 0.5);
@@ -500,31 +508,55 @@ __device__ Rows solid_triangle(const VoxParams& p, const float* t, Sink* out, co
     // the table (no-ops for meshes inside the box)
     int y0 = (int)fmaxf(gmin_y, -1.0f), y1 = (int)fminf(gmax_y, (float)p.G);
     int z0 = (int)fmaxf(gmin_z, -1.0f), z1 = (int)fminf(gmax_z, (float)p.G);
-    y0 = max(y0, 0); z0 = max(z0, 0); y1 = min(y1, G1); z1 = min(z1, G1);
-    if (ROWS_ONLY) return {y0, y1, z0, z1};
+    st.a = a;
+    st.b = b;
+    st.c = c;
+    st.n = n;
+    st.v0 = v0;
+    st.y0 = max(y0, 0);
+    st.z0 = max(z0, 0);
+    st.y1 = min(y1, G1);
+    st.z1 = min(z1, G1);
+    st.top_left = (top_left_edge(a, b) ? 1u : 0u) | (top_left_edge(b, c) ? 2u : 0u) | (top_left_edge(c, a) ? 4u : 0u);
+    return true;
+}
 
-    for (int y = max(y0, clip.y0); y <= min(y1, clip.y1); ++y) {
-        for (int z = max(z0, clip.z0); z <= min(z1, clip.z1); ++z) {
-            const f2 pt = {(float)(((double)y + 0.5) * (double)uy), (float)(((double)z + 0.5) * (double)uz)};
-            const int k = check_point_triangle(a, b, c, pt);
-            if ((k == 1 && top_left_edge(a, b)) || (k == 2 && top_left_edge(b, c)) ||
-                (k == 3 && top_left_edge(c, a)) || (k == 0)) {
-                const float xc = (-(n.y * (pt.x - v0.y) + n.z * (pt.y - v0.z)) / n.x + v0.x);
-                const double xq = (double)(xc / ux) - 0.5;
-                if (!(xq > -1.0)) continue;
-                const int xmax = (xq >= (double)G1) ? G1 : (int)xq;
-                if (p.morton) {
-                    for (int x = 0; x <= xmax; ++x) out->set(vox_index(p, x, y, z), z);
-                    out->flush();
-                } else {
-                    const uint64_t row = vox_index(p, 0, y, z);
-                    out->set_range(row, row + (uint64_t)xmax, z);
-                }
-            }
-        }
+// One sample row (y, z): the parity test of cpu_voxelizer.cpp:293-320.
+template <class Sink>
+__device__ __forceinline__ void solid_sample(const VoxParams& p, const SolidTri& st, int y, int z, Sink* out) {
+    const float uy = p.unit[1], uz = p.unit[2], ux = p.unit[0];
+    const f2 pt = {(float)(((double)y + 0.5) * (double)uy), (float)(((double)z + 0.5) * (double)uz)};
+    const int k = check_point_triangle(st.a, st.b, st.c, pt);
+    if (!((k == 1 && (st.top_left & 1u)) || (k == 2 && (st.top_left & 2u)) || (k == 3 && (st.top_left & 4u)) ||
+          (k == 0)))
+        return;
+    const f3 n = st.n, v0 = st.v0;
+    const float xc = (-(n.y * (pt.x - v0.y) + n.z * (pt.y - v0.z)) / n.x + v0.x);
+    const double xq = (double)(xc / ux) - 0.5;
+    if (!(xq > -1.0)) return;
+    const int G1 = (int)p.G - 1;
+    const int xmax = (xq >= (double)G1) ? G1 : (int)xq;
+    if (p.morton) {
+        for (int x = 0; x <= xmax; ++x) out->set(vox_index(p, x, y, z), z);
+        out->flush();
+    } else {
+        const uint64_t row = vox_index(p, 0, y, z);
+        out->set_range(row, row + (uint64_t)xmax, z);
     }
+}
+
+// cpu_voxelizer.cpp:265-325 for one triangle, restricted to sample rows y in
+// [clip.y0, clip.y1] and z in [clip.z0, clip.z1].  ROWS_ONLY: return the rows
+// the walk would visit instead (empty for triangles that set nothing).
+template <bool ROWS_ONLY, class Sink>
+__device__ Rows solid_triangle(const VoxParams& p, const float* t, Sink* out, const Rows clip) {
+    SolidTri st;
+    if (!solid_setup(p, t, st)) return {1, 0, 1, 0};
+    if (ROWS_ONLY) return {st.y0, st.y1, st.z0, st.z1};
+    for (int y = max(st.y0, clip.y0); y <= min(st.y1, clip.y1); ++y)
+        for (int z = max(st.z0, clip.z0); z <= min(st.z1, clip.z1); ++z) solid_sample(p, st, y, z, out);
     out->flush();
-    return {y0, y1, z0, z1};
+    return {st.y0, st.y1, st.z0, st.z1};
 }
 
 constexpr int kMaxTiles = 9;            // triangles touching more tiles take the direct path
@@ -833,13 +865,56 @@ __global__ __launch_bounds__(kVoxThreads) void tile_vox_kernel(VoxParams p, cons
         for (uint32_t k = threadIdx.x; k < tile_words; k += kVoxThreads) lds[k] = 0;
         __syncthreads();
         if constexpr (SOLID) {
-            for (uint32_t k = begin + threadIdx.x; k < end; k += kVoxThreads) {
-                const uint64_t i = idx[k];
-                float t[9];
+            // A batch of kVoxThreads triangles: each lane sets up one and counts its
+            // sample rows inside the tile; a block scan numbers all the samples, and
+            // lanes then take samples in turn from any triangle of the batch, so a
+            // wave runs the batch's mean sample count, not its largest triangle's.
+            // After the tile: set-up triangles, sample offsets, scan partials.
+            SolidTri* s_st = reinterpret_cast<SolidTri*>(lds + tile_words);
+            uint32_t* s_off = reinterpret_cast<uint32_t*>(s_st + kVoxThreads);  // kVoxThreads + 1
+            uint32_t* s_part = s_off + kVoxThreads + 1;
+            LdsSink<true, DIFF> out{lds, (uint64_t)p.G, clip.z0, clip.y0, geo.wpp};
+            for (uint32_t kb = begin; kb < end; kb += kVoxThreads) {
+                const uint32_t k = kb + threadIdx.x;
+                uint32_t cnt = 0;
+                if (k < end) {
+                    const uint64_t i = idx[k];
+                    float t[9];
 #pragma unroll
-                for (int j = 0; j < 9; ++j) t[j] = tris[i * 9 + j];
-                LdsSink<true, DIFF> out{lds, (uint64_t)p.G, clip.z0, clip.y0, geo.wpp};
-                solid_triangle<false>(p, t, &out, clip);
+                    for (int j = 0; j < 9; ++j) t[j] = tris[i * 9 + j];
+                    SolidTri st;
+                    if (solid_setup(p, t, st)) {
+                        st.y0 = max(st.y0, clip.y0);
+                        st.y1 = min(st.y1, clip.y1);
+                        st.z0 = max(st.z0, clip.z0);
+                        st.z1 = min(st.z1, clip.z1);
+                        if (st.y0 <= st.y1 && st.z0 <= st.z1) {
+                            cnt = (uint32_t)(st.y1 - st.y0 + 1) * (uint32_t)(st.z1 - st.z0 + 1);
+                            s_st[threadIdx.x] = st;
+                        }
+                    }
+                }
+                uint32_t excl;
+                const uint32_t total = block_exclusive_scan(cnt, s_part, excl);
+                s_off[threadIdx.x] = excl;
+                if (threadIdx.x == 0) s_off[kVoxThreads] = total;
+                __syncthreads();
+                for (uint32_t si = threadIdx.x; si < total; si += kVoxThreads) {
+                    uint32_t lo = 0, hi = kVoxThreads;  // s_off[lo] <= si < s_off[hi]
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (s_off[mid] <= si) lo = mid;
+                        else hi = mid;
+                    }
+                    const SolidTri st = s_st[lo];
+                    const uint32_t local = si - s_off[lo];
+                    const uint32_t nz = (uint32_t)(st.z1 - st.z0 + 1);
+                    // local / nz: local < 64 nz and nz <= 64 (tiles are at most 22 rows
+                    // a side), where a float reciprocal within 2 ulp floors exactly
+                    const uint32_t yi = (uint32_t)(((float)local + 0.5f) * __frcp_rn((float)nz));
+                    solid_sample(p, st, st.y0 + (int)yi, st.z0 + (int)(local - yi * nz), &out);
+                }
+                __syncthreads();  // the batch's set-ups and offsets are reused by the next
             }
         } else {
             // after the tile: set-up triangles, column prefix, scan partials
@@ -1012,7 +1087,7 @@ bool tile_geometry(uint32_t G, uint64_t n_tri, bool force, bool accumulate, Tile
     uint64_t T = std::min<uint64_t>(G, kMinTile);
     while ((G + T - 1) / T * ((G + T - 1) / T) > kMaxTileCount) ++T;
     const uint64_t wpp = aligned ? T * G / 32 : (T * G + 31) / 32 + 1;
-    if (T * wpp > kTileWords) return false;
+    if (T * wpp > kTileWords || T > 64) return false;  // (T <= 22 follows from the word limit)
     const uint64_t tiles_y = (G + T - 1) / T;
     g.T = (uint32_t)T;
     g.tiles_y = (uint32_t)tiles_y;
@@ -1095,8 +1170,9 @@ void launch_voxelize(hipStream_t stream, const VoxParams& p, const TileGeo& geo,
     hipLaunchKernelGGL(tile_scatter_kernel, dim3(n_blocks), dim3(kVoxThreads), scatter_smem, stream, p.n_tri, geo,
                        n_blocks, routes, counts, start, idx);
     NLOD_LAUNCH_CHECK();
-    size_t tile_smem = (size_t)geo.T * geo.wpp * sizeof(uint32_t);
-    if (!SOLID) tile_smem += kVoxThreads * sizeof(SurfTri) + (kVoxThreads + 1 + kVoxThreads / 64) * sizeof(uint32_t);
+    size_t tile_smem = (size_t)geo.T * geo.wpp * sizeof(uint32_t) +
+                       (kVoxThreads + 1 + kVoxThreads / 64) * sizeof(uint32_t);
+    tile_smem += kVoxThreads * (SOLID ? sizeof(SolidTri) : sizeof(SurfTri));
     auto run_tiles = [&](auto kernel) {
         const int per_cu = occupancy_blocks(reinterpret_cast<const void*>(kernel), kVoxThreads, tile_smem);
         const uint32_t grid = (uint32_t)(cu_count() * per_cu);
