@@ -111,17 +111,16 @@ __device__ __forceinline__ void load_entry(const TT* __restrict__ p, float* out)
     }
 }
 
-// One level of a grid for one query: F features interpolated in FP32.
+// One level of a grid for one query: F features interpolated in FP32, from the
+// level's own entries at tab (global memory, or a copy in LDS).
 template <int F, typename TT = float>
-__device__ __forceinline__ void grid_level(const GridLevels& lv, uint32_t level, float x0, float x1, float x2,
-                                           const TT* __restrict__ table, float* v) {
+__device__ __forceinline__ void grid_level_at(const GridLevels& lv, uint32_t level, float x0, float x1, float x2,
+                                              const TT* __restrict__ tab, float* v) {
     // no FMA contraction: the encoding rounds identically in every kernel it is inlined into
 #pragma clang fp contract(off)
     const float scale = lv.scale[level];
     const uint32_t res = lv.res[level];
-    const uint32_t off = lv.offset[level];
-    const uint32_t hs = lv.offset[level + 1] - off;
-    const TT* tab = table + (size_t)off * F;
+    const uint32_t hs = lv.offset[level + 1] - lv.offset[level];
     float fr[3];
     uint32_t cell[3];
     grid_pos(x0, scale, lv.interp, fr[0], cell[0]);
@@ -149,6 +148,13 @@ __device__ __forceinline__ void grid_level(const GridLevels& lv, uint32_t level,
 #pragma unroll
         for (int f = 0; f < F; ++f) v[f] += w * vals[c][f];
     }
+}
+
+// One level of a grid for one query from the whole table.
+template <int F, typename TT = float>
+__device__ __forceinline__ void grid_level(const GridLevels& lv, uint32_t level, float x0, float x1, float x2,
+                                           const TT* __restrict__ table, float* v) {
+    grid_level_at<F, TT>(lv, level, x0, x1, x2, table + (size_t)lv.offset[level] * F, v);
 }
 
 // Entry e (0 <= e < 8 / F) of a 16-byte block of FP16 entries with F features.

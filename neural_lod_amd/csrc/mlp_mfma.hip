@@ -48,6 +48,19 @@ struct LevelMap {
     uint8_t level[kMaxLevelRows];
 };
 
+// Grid levels whose FP16 tables together fit in LDS (the coarse dense levels):
+// encoded by encode_lds_kernel instead of rows of encode_kernel.
+constexpr int kMaxLdsLevels = 4;
+constexpr uint32_t kLdsTableBytes = 72 * 1024;  // two blocks per CU
+constexpr int kLdsBlock = 1024;
+struct LdsLevels {
+    uint32_t n;
+    uint32_t halves;                  // FP16 values staged (all levels)
+    uint8_t seg[kMaxLdsLevels];
+    uint8_t level[kMaxLdsLevels];
+    uint32_t at[kMaxLdsLevels];       // first staged half of each level
+};
+
 // pair of 2 halves holding feature f of query q
 __device__ __forceinline__ uint32_t* pair_ptr(half_t* feat, uint32_t q, uint32_t f, uint32_t kp2) {
     return reinterpret_cast<uint32_t*>(feat + feat_pair_offset(q >> 4, f >> 1, kp2, q & 15));
@@ -114,6 +127,57 @@ __global__ __launch_bounds__(kBlock) void encode_kernel(uint32_t N, uint32_t N16
                 half_t* d = feat + feat_pair_offset(q >> 4, f >> 1, kp2, q & 15);
                 if (!(m2 & 1u)) d[0] = a0;
                 if (!(m2 & 2u)) d[1] = a1;
+            }
+        }
+    }
+}
+
+// The levels of `lv` (at most kMaxLdsLevels, tables in LDS): one block copies
+// their FP16 tables into LDS once, then each lane encodes a grid-stride run of
+// queries, reading the query's position once for all of them.  The gathers come
+// from LDS instead of the L2, which every other level row keeps busy.
+template <int F>
+__global__ __launch_bounds__(kLdsBlock) void encode_lds_kernel(uint32_t N, uint32_t N16, LdsLevels lv,
+                                                               const EncProgram* __restrict__ prog,
+                                                               const half_t* __restrict__ table,
+                                                               const float* __restrict__ in,
+                                                               half_t* __restrict__ feat) {
+    extern __shared__ uint4 lds_tab4[];
+    half_t* lt = reinterpret_cast<half_t*>(lds_tab4);
+    const EncProgram& P = *prog;
+    for (uint32_t l = 0; l < lv.n; ++l) {
+        const DevSegment& s = P.segs[lv.seg[l]];
+        const GridLevels& g = s.lv;
+        const uint32_t n = (g.offset[lv.level[l] + 1] - g.offset[lv.level[l]]) * F;
+        const half_t* src = table + s.param_offset + (size_t)g.offset[lv.level[l]] * F;
+        if ((reinterpret_cast<uintptr_t>(src) & 15u) == 0 && lv.at[l] % 8 == 0 && n % 8 == 0) {
+            const uint4* s4 = reinterpret_cast<const uint4*>(src);
+            uint4* d4 = reinterpret_cast<uint4*>(lt + lv.at[l]);
+            for (uint32_t k = threadIdx.x; k < n / 8; k += kLdsBlock) d4[k] = s4[k];
+        } else {
+            for (uint32_t k = threadIdx.x; k < n; k += kLdsBlock) lt[lv.at[l] + k] = src[k];
+        }
+    }
+    __syncthreads();
+    const uint32_t kp2 = P.k_padded / 2;
+    for (uint32_t q = blockIdx.x * kLdsBlock + threadIdx.x; q < N16; q += gridDim.x * kLdsBlock) {
+        const bool valid = q < N;
+        const float* x = in + (size_t)(valid ? q : 0) * P.n_input_dims;
+        for (uint32_t l = 0; l < lv.n; ++l) {
+            const DevSegment& s = P.segs[lv.seg[l]];
+            const uint32_t level = lv.level[l];
+            const float* xs = x + s.in_offset;
+            float v[F];
+            grid_level_at<F, half_t>(s.lv, level, xs[0], xs[1], xs[2], lt + lv.at[l], v);
+            const uint32_t f0 = s.out_offset + level * F;  // multiple of F
+            half_t h[F];
+#pragma unroll
+            for (int f = 0; f < F; ++f) h[f] = valid ? (half_t)v[f] : (half_t)0.0f;
+            if constexpr (F == 1) {
+                feat[feat_pair_offset(q >> 4, f0 >> 1, kp2, q & 15) + (f0 & 1)] = h[0];
+            } else {
+#pragma unroll
+                for (int j = 0; j < F / 2; ++j) *pair_ptr(feat, q, f0 + 2 * j, kp2) = pack2(h[2 * j], h[2 * j + 1]);
             }
         }
     }
@@ -265,19 +329,30 @@ __global__ void to_half_kernel(const float* __restrict__ src, half_t* __restrict
         dst[i] = (half_t)src[i];
 }
 
-LevelMap make_level_map(const Layout& L) {
-    LevelMap m{};
+// Splits the grid levels between the LDS pass (levels whose tables still fit,
+// in order) and the rows of the per-level pass (the rest).
+void make_level_maps(const Layout& L, LevelMap& m, LdsLevels& d) {
+    m = LevelMap{};
+    d = LdsLevels{};
     for (size_t si = 0; si < L.segs.size(); ++si) {
         const Segment& s = L.segs[si];
         if (s.type != SEG_GRID) continue;
         for (uint32_t l = 0; l < s.n_levels; ++l) {
+            const uint32_t halves = (s.levels.offset[l + 1] - s.levels.offset[l]) * s.F;
+            if (d.n < (uint32_t)kMaxLdsLevels && (size_t)(d.halves + halves) * sizeof(half_t) <= kLdsTableBytes) {
+                d.seg[d.n] = (uint8_t)si;
+                d.level[d.n] = (uint8_t)l;
+                d.at[d.n] = d.halves;
+                d.halves += halves;
+                ++d.n;
+                continue;
+            }
             if (m.n >= (uint32_t)kMaxLevelRows) fail(NLOD_ERR_UNSUPPORTED, "more than 64 grid levels in one model");
             m.seg[m.n] = (uint8_t)si;
             m.level[m.n] = (uint8_t)l;
             ++m.n;
         }
     }
-    return m;
 }
 
 }  // namespace
@@ -314,8 +389,23 @@ void mfma_to_half(hipStream_t stream, const float* src, void* dst, size_t n) {
 void mfma_encode(hipStream_t stream, const Layout& L, const void* d_program, const void* table16, const float* in,
                  uint32_t N, void* feat) {
     if (N == 0) return;
-    const LevelMap map = make_level_map(L);
+    LevelMap map;
+    LdsLevels lds;
+    make_level_maps(L, map, lds);
     const uint32_t N16 = div_round_up(N, 16) * 16;
+    if (lds.n) {
+        const size_t smem = (size_t)lds.halves * sizeof(half_t);
+        const uint32_t blocks = std::min<uint32_t>(div_round_up(N16, kLdsBlock), (uint32_t)cu_count() * 2);
+        dispatch_fused(L, [&](auto Wc, auto Fc, auto Dc) {
+            constexpr int F = decltype(Fc)::value;
+            NLOD_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(encode_lds_kernel<F>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsTableBytes));
+            hipLaunchKernelGGL(encode_lds_kernel<F>, dim3(blocks), dim3(kLdsBlock), smem, stream, N, N16, lds,
+                               static_cast<const EncProgram*>(d_program), static_cast<const half_t*>(table16), in,
+                               static_cast<half_t*>(feat));
+        });
+        NLOD_LAUNCH_CHECK();
+    }
     const dim3 grid(div_round_up(2 * N16, kBlock), map.n + 1);  // grid rows take two lanes per query
     dispatch_fused(L, [&](auto Wc, auto Fc, auto Dc) {
         constexpr int F = decltype(Fc)::value, D = decltype(Dc)::value;
