@@ -1,14 +1,17 @@
 // FP16-MFMA form of the encode + fully fused MLP (gfx950).
 //
-// Two launches per batch:
-//  1. Encoding.  Two lanes per (query, grid level), one for each x-side of the
-//     cell (grid_level_h16_pair: the two corners of an x-edge then cost one L2
-//     request), blockIdx.y = level, so
-//     the workgroups resident at any time all gather from one level's table:
-//     that level (2 MiB as FP16 at 2^19 entries x 2 features) stays in each
-//     XCD's 4 MiB L2 while 2^20 queries stream through it (the reference's
-//     "one level per block" idea, grid.h:1053-1058).  The last blockIdx.y row
-//     writes the non-grid features (spherical harmonics, identity, padding).
+// Three launches per batch:
+//  0. The coarse levels whose FP16 tables fit in LDS together (72 KB: levels 0
+//     and 1 of the 16^3-based grids): encode_lds_kernel gathers from a copy of
+//     those tables in LDS, sparing the L2 that the other levels saturate.
+//  1. The other levels.  Two lanes per (query, grid level), one for each x-side
+//     of the cell (grid_level_h16_pair: the two corners of an x-edge then cost
+//     one L2 request), blockIdx.y = level, so the workgroups resident at any
+//     time all gather from one level's table: that level (2 MiB as FP16 at 2^19
+//     entries x 2 features) stays in each XCD's 4 MiB L2 while 2^20 queries
+//     stream through it (the reference's "one level per block" idea,
+//     grid.h:1053-1058).  The last blockIdx.y row writes the non-grid features
+//     (spherical harmonics, identity, padding), overlapping the level rows.
 //     Features are stored FP16 as [N/16][K/2][16 queries][2 features]: one
 //     level of a 16-query tile is one contiguous 64-byte row, written whole
 //     (a layout with 8 features per query row was written a quarter line at a
