@@ -97,10 +97,13 @@ def test_inference_fp32(cuda, name):
         _fp32_close(out, ref, f"inference {name} n={n}")
 
 
-@pytest.mark.parametrize("name", ["visibility", "throughput", "reference_visibility"])
+@pytest.mark.parametrize("name", ["visibility", "throughput", "reference_visibility", "dense_smoothstep", "hash_f8"])
 @pytest.mark.parametrize("grid_range", [1e-4, 1.0])
 def test_inference_fp16_mfma(cuda, name, grid_range):
-    cfg = configs.ALL[name]
+    """FP16 MFMA form vs the FP64 oracle.  dense_smoothstep (F = 4, all levels
+    small enough for the LDS pass) and hash_f8 (F = 8, 64 KB levels: one in the
+    LDS pass, the rest in the per-level pass) cover the other feature widths."""
+    cfg = ALL_CFGS[name]
     print(f"case inference fp16 {name} grid_range={grid_range}", flush=True)
     m = _model(cfg, precision=1)
     L, p = _params(cfg, m, 5, grid_range=grid_range)
