@@ -35,10 +35,10 @@ ENC_BYTES_PER_QUERY = 16 * 8 * 2 * 2 + 6 * 4 + 48 * 2
 # whole step (encode + MLP): + features read back + 1 x 4 B output
 BYTES_PER_QUERY = ENC_BYTES_PER_QUERY + 48 * 2 + 4
 FLOP_PER_QUERY = 2 * (48 * 64 + 3 * 64 * 64 + 64 * 1)
-# HBM bytes per launch of the encoding pass at 2^20 queries, from rocprofv3
-# FETCH_SIZE and WRITE_SIZE passes calibrated by a 256 MiB copy
-# (bench.py --profile-encode; 604.3 MB fetched + 134.8 MB written)
-ENC_TRAFFIC_BYTES = 739.1e6
+# HBM bytes of one encoding pass (its two launches) at 2^20 queries, from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE passes calibrated by a 256 MiB copy
+# (bench.py --profile-encode; 601.3 MB fetched + 135.9 MB written)
+ENC_TRAFFIC_BYTES = 737.2e6
 
 
 def log(msg):
@@ -472,7 +472,7 @@ def main():
         "speedup_vs_plain_fp32": results["fp32"]["ms"] / fp16["ms"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": ENC_TRAFFIC_BYTES,
-                     "kernel": "encode_kernel<2,4,half> (per-level hash-grid gathers)",
+                     "kernel": "FP16 encoding pass: encode_lds_kernel<2> (levels 0-1 from LDS) + encode_kernel<2,4,half> (levels 2-15 and SH)",
                      "kernel_ms": enc_ms, "bytes_per_query": ENC_BYTES_PER_QUERY,
                      "mlp_kernel_ms": fp16["ms"] - enc_ms, "mlp_flop_per_query": FLOP_PER_QUERY,
                      "mlp_tflops": FLOP_PER_QUERY * N_QUERIES / ((fp16["ms"] - enc_ms) * 1e-3) / 1e12},
