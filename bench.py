@@ -129,6 +129,7 @@ LABEL_RES = 64          # LoD grid of the visibility training labels
 VOX_RES = 256
 VOX_TRIS = 200_000       # the CPU-runnable voxelization workload (blob mesh, 256^3 solid)
 VOX_TRIS_LARGE = 20_000_000
+VOX_TRIS_SWEEP = ((2_000_000, "2M"), (VOX_TRIS_LARGE, "20M"))  # larger meshes of the same construction
 
 
 THR_RES = 256     # finest level of the throughput-target pyramid (256^3 .. 16^3)
@@ -198,8 +199,8 @@ def throughput_train_bench(dev, args):
 def voxelize_bench(dev, args):
     """Solid and surface voxelization of the seeded 200k-triangle blob mesh at
     256^3 (bit-exact check against the CPU restatement, which is also timed as
-    the CPU baseline), plus a 20M-triangle mesh of the same construction for
-    the streaming rate.  Bytes per mesh: 36 B per triangle read + the table
+    the CPU baseline), plus 2M- and 20M-triangle meshes of the same
+    construction for the rate at scale.  Bytes per mesh: 36 B per triangle read + the table
     written once."""
     import ctypes as C
     import torch
@@ -246,15 +247,16 @@ def voxelize_bench(dev, args):
                                                    "(C restatement of cpu_voxelizer.cpp, one thread)"}
         log(f"voxelize {mode} 200k @ {VOX_RES}^3: {ms * 1e3:.1f} us  {out[mode]}")
     del tris
-    big = torch.from_numpy(synthetic.blob_mesh(VOX_TRIS_LARGE, seed=0)).to(dev)
-    for mode, solid in (("solid", True), ("surface", False)):
-        ms, _ = time_vox(big, solid, 3)
-        nbytes = VOX_TRIS_LARGE * 36 + table_bytes
-        out[mode + "_20M"] = {"ms_per_mesh": ms, "triangles_per_s": VOX_TRIS_LARGE / (ms * 1e-3),
-                              "achieved_GBs": nbytes / (ms * 1e-3) / 1e9,
-                              "frac_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
-        log(f"voxelize {mode} 20M @ {VOX_RES}^3: {ms:.3f} ms")
-    del big
+    for n_big, tag in VOX_TRIS_SWEEP:
+        big = torch.from_numpy(synthetic.blob_mesh(n_big, seed=0)).to(dev)
+        for mode, solid in (("solid", True), ("surface", False)):
+            ms, _ = time_vox(big, solid, 3)
+            nbytes = n_big * 36 + table_bytes
+            out[mode + "_" + tag] = {"ms_per_mesh": ms, "triangles_per_s": n_big / (ms * 1e-3),
+                                     "achieved_GBs": nbytes / (ms * 1e-3) / 1e9,
+                                     "frac_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            log(f"voxelize {mode} {tag} @ {VOX_RES}^3: {ms:.3f} ms")
+        del big
     torch.cuda.empty_cache()
     return out
 
