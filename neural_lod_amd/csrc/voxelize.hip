@@ -643,13 +643,17 @@ __global__ __launch_bounds__(kVoxThreads) void vox_solid_kernel(VoxParams p, con
     }
 }
 
-// Direct path, surface: one triangle per lane, swept along x so that the
-// bits of one row merge into one global atomic per word.  routes (nullable):
-// only the triangles routed to the direct path.
+// Direct path, surface: swept along x so that the bits of one row merge into
+// one global atomic per word.  Each lane sets up one triangle of a batch and
+// the batch's x columns are spread over the block (surface_block), so a wave
+// does not wait on its largest triangle.  routes (nullable): only the
+// triangles routed to the direct path, one per lane.
 __global__ __launch_bounds__(kVoxThreads) void vox_surface_kernel(VoxParams p, const float* __restrict__ tris,
                                                                   const uint64_t* __restrict__ routes,
                                                                   uint32_t* __restrict__ table) {
     __shared__ float s_tri[kVoxThreads * 9];
+    __shared__ SurfTri s_st[kVoxThreads];
+    __shared__ uint32_t s_pre[kVoxThreads + 1], s_part[kVoxThreads / 64];
     const Rows all{0, (int)p.G - 1, 0, (int)p.G - 1};
     for (uint64_t base = (uint64_t)blockIdx.x * kVoxThreads; base < p.n_tri;
          base += (uint64_t)gridDim.x * kVoxThreads) {
@@ -671,13 +675,25 @@ __global__ __launch_bounds__(kVoxThreads) void vox_surface_kernel(VoxParams p, c
             }
             __syncthreads();
         }
-        if (!have) continue;
-        const SurfTri st = surface_setup(p, t, all, true);
-        if (st.axis != 0) continue;
+        if (routes) {
+            // the few triangles the tiled path hands over: one per lane
+            if (!have) continue;
+            const SurfTri st = surface_setup(p, t, all, true);
+            if (st.axis != 0) continue;
+            GlobalSink<false> out{table};
+            const uint32_t cols = surface_columns<0>(st);
+            for (uint32_t j = 0; j < cols; ++j) surface_column<0>(p, st, j, out);
+            out.flush();
+            continue;
+        }
+        // a whole batch: the x columns of its 256 triangles spread over the block
+        SurfTri st;
+        st.axis = -1;
+        if (have) st = surface_setup(p, t, all, true);
+        s_st[threadIdx.x] = st;
+        __syncthreads();
         GlobalSink<false> out{table};
-        const uint32_t cols = surface_columns<0>(st);
-        for (uint32_t j = 0; j < cols; ++j) surface_column<0>(p, st, j, out);
-        out.flush();
+        surface_block(p, s_st, s_pre, s_part, out);
     }
 }
 
