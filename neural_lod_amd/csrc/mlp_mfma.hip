@@ -73,6 +73,73 @@ __device__ __forceinline__ uint32_t pack2(half_t a, half_t b) {
     return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
 }
 
+// Non-grid rows of query q, each computed once and written in the pair layout:
+// SH (pad rows of 1.0 first, then the basis), identity (inputs scaled, then pad
+// rows of 1.0), the pad rows after a grid (0) and the K tail (0).  Same values,
+// bit for bit, as encode_window_f32 gives these rows.
+template <int MAXD>
+__device__ __forceinline__ void encode_nongrid(const EncProgram& P, const float* __restrict__ x, uint32_t q,
+                                               bool valid, half_t* __restrict__ feat) {
+#pragma clang fp contract(off)
+    // row r of query q sits at base + (r / 2) * 32 + r % 2 (feat_pair_offset)
+    half_t* base = feat + feat_pair_offset(q >> 4, 0, P.k_padded / 2, q & 15);
+    // Each value is rounded to FP32 first, then to FP16, as everywhere else: the
+    // empty asm keeps the compiler from folding the product that made it into
+    // the conversion (v_fma_mix, a single rounding).  It emits no instruction.
+    auto h = [&](float v) -> half_t {
+        asm volatile("" : "+v"(v));
+        return valid ? (half_t)v : (half_t)0.0f;
+    };
+    auto put = [&](uint32_t r, float v) { base[(r >> 1) * 32 + (r & 1)] = h(v); };
+    auto put2 = [&](uint32_t r, float a, float b) {  // r even
+        *reinterpret_cast<uint32_t*>(base + (r >> 1) * 32) = pack2(h(a), h(b));
+    };
+    for (uint32_t si = 0; si < P.n_segs; ++si) {
+        const DevSegment& s = P.segs[si];
+        const uint32_t r0 = s.out_offset;
+        if (s.type == SEG_GRID) {
+            for (uint32_t k = 0; k < s.n_pad; ++k) put(r0 + s.n_out + k, 0.0f);
+            continue;
+        }
+        const float* xs = x + s.in_offset;
+        if (s.type == SEG_SH) {
+            for (uint32_t k = 0; k < s.n_pad; ++k) put(r0 + k, 1.0f);
+            const uint32_t sh0 = r0 + s.n_pad, nsh = s.degree * s.degree;
+            const float vx = xs[0] * 2.f - 1.f, vy = xs[1] * 2.f - 1.f, vz = xs[2] * 2.f - 1.f;
+            float X[MAXD], Y[MAXD], Z[MAXD];
+            X[0] = Y[0] = Z[0] = 1.0f;
+#pragma unroll
+            for (int k = 1; k < MAXD; ++k) {
+                X[k] = X[k - 1] * vx;
+                Y[k] = Y[k - 1] * vy;
+                Z[k] = Z[k - 1] * vz;
+            }
+            float v[MAXD * MAXD];
+#pragma unroll
+            for (int k = 0; k < MAXD * MAXD; ++k) v[k] = 0.0f;
+            sh_basis<MAXD>(s.degree, X, Y, Z, [&](int k, float val) { v[k] = val; });
+            if ((sh0 & 1u) == 0) {
+#pragma unroll
+                for (int k = 0; k < MAXD * MAXD; k += 2) {
+                    if ((uint32_t)k + 1 < nsh) put2(sh0 + k, v[k], v[k + 1]);
+                    else if ((uint32_t)k < nsh) put(sh0 + k, v[k]);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < MAXD * MAXD; ++k)
+                    if ((uint32_t)k < nsh) put(sh0 + k, v[k]);
+            }
+        } else {  // identity
+            for (uint32_t k = 0; k < s.n_out; ++k) put(r0 + k, xs[k] * s.id_scale + s.id_offset);
+            for (uint32_t k = 0; k < s.n_pad; ++k) put(r0 + s.n_out + k, 1.0f);
+        }
+    }
+    uint32_t r = P.enc_width;
+    if (r & 1u) put(r++, 0.0f);
+    for (; r + 1 < P.k_padded; r += 2) put2(r, 0.0f, 0.0f);
+    if (r < P.k_padded) put(r, 0.0f);
+}
+
 template <int F, int MAXD, typename TT>
 __global__ __launch_bounds__(kBlock) void encode_kernel(uint32_t N, uint32_t N16, LevelMap map,
                                                         const EncProgram* __restrict__ prog,
@@ -80,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void encode_kernel(uint32_t N, uint32_t N16
                                                         half_t* __restrict__ feat) {
     static_assert(std::is_same<TT, half_t>::value, "the encoding pass gathers from the FP16 table copy");
     const EncProgram& P = *prog;
-    const uint32_t kp2 = P.k_padded / 2, kp8 = P.k_padded / 8;
+    const uint32_t kp2 = P.k_padded / 2;
     if (blockIdx.y < map.n) {
         // grid rows: two lanes per query (grid_level_h16_pair); N16 is a multiple
         // of 16, so both lanes of a pair return together
@@ -111,28 +178,7 @@ __global__ __launch_bounds__(kBlock) void encode_kernel(uint32_t N, uint32_t N16
     const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
     if (q >= N16) return;
     const bool valid = q < N;
-    const float* x = in + (size_t)(valid ? q : 0) * P.n_input_dims;
-    // non-grid rows: SH, identity, padding (grid pad rows and the K tail are 0)
-    for (uint32_t g8 = 0; g8 < kp8; ++g8) {
-        const uint32_t mask = P.grid_rows[g8];
-        if (mask == 0xffu) continue;
-        float win[8];
-        encode_window_f32<F, MAXD, TT, true>(P, table, x, 8 * g8, win);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const uint32_t m2 = (mask >> (2 * j)) & 3u;
-            const half_t a0 = valid ? (half_t)win[2 * j] : (half_t)0.0f;
-            const half_t a1 = valid ? (half_t)win[2 * j + 1] : (half_t)0.0f;
-            const uint32_t f = 8 * g8 + 2 * j;
-            if (m2 == 0) {
-                *pair_ptr(feat, q, f, kp2) = pack2(a0, a1);
-            } else if (m2 != 3u) {  // one row of the pair belongs to a grid level
-                half_t* d = feat + feat_pair_offset(q >> 4, f >> 1, kp2, q & 15);
-                if (!(m2 & 1u)) d[0] = a0;
-                if (!(m2 & 2u)) d[1] = a1;
-            }
-        }
-    }
+    encode_nongrid<MAXD>(P, in + (size_t)(valid ? q : 0) * P.n_input_dims, q, valid, feat);
 }
 
 // The levels of `lv` (at most kMaxLdsLevels, tables in LDS): one block copies
