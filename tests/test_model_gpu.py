@@ -117,6 +117,34 @@ def test_inference_fp16_mfma(cuda, name, grid_range):
         assert err <= 2e-3, f"FP16 MFMA {name} n={n}: max abs error {err:.3e} > 2e-3"
 
 
+@pytest.mark.parametrize("name", ["visibility", "throughput", "reference_visibility", "dense_smoothstep", "hash_f8"])
+def test_encoding_pass_features_fp16(cuda, name):
+    """The FP16 encoding pass feature by feature (coarse levels from the LDS
+    pass, the other levels from the per-level rows, SH / identity / pad rows
+    from the non-grid row) against the FP64 oracle's encoding of the same
+    FP16-rounded tables: within FP16 rounding of each value, and the K padding
+    beyond the encoded width exactly 0."""
+    cfg = ALL_CFGS[name]
+    print(f"case encoding pass features {name}", flush=True)
+    m = _model(cfg, precision=1)
+    L, p = _params(cfg, m, 11, grid_range=1.0)
+    m.set_params(torch.from_numpy(p).cuda())
+    n = 3000  # not a multiple of 16: the last tile is partly padding
+    x = _inputs(cfg, n, 12)
+    kp = (L.net.n_input + 31) // 32 * 32
+    buf = m.encode_fragments(torch.from_numpy(x).cuda()).view(torch.float16).cpu().numpy().astype(np.float64)
+    tiles = (n + 15) // 16
+    got = buf[:tiles * kp * 16].reshape(tiles, kp // 2, 16, 2).transpose(0, 2, 1, 3).reshape(tiles * 16, kp)
+    p16 = p.astype(np.float16).astype(np.float32)  # the table copy the encoding pass gathers from
+    ref = M.encode(L, p16, x)
+    w = ref.shape[1]
+    err = np.abs(got[:n, :w] - ref)
+    bound = 2.0 ** -10 * np.abs(ref) + 1e-6  # FP16 rounding of the result, FP32 interpolation
+    assert np.all(err <= bound), f"{name}: {np.sum(err > bound)} features outside FP16 rounding, max {err.max():.3e}"
+    assert not np.any(got[:n, w:]), f"{name}: K padding rows are not 0"
+    assert not np.any(got[n:]), f"{name}: padding queries of the last tile are not 0"
+
+
 def test_fp16_matches_fp32_form(cuda):
     cfg = configs.VISIBILITY
     m16, m32 = _model(cfg, 1), _model(cfg, 0)
