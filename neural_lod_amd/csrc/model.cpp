@@ -243,8 +243,10 @@ uint32_t Model::train_fused(hipStream_t s, uint32_t N, const float* in, const fl
 // Parameter initialisation of tiny-cuda-nn's Trainer (trainer.h:53-92):
 // pcg32 seeded from std::seed_seq{seed}; network matrices Xavier-uniform on the
 // host in order (gpu_matrix.h:291-306), then each grid table U[-1e-4, 1e-4]
-// drawn as generate_random_uniform does it (random.h:63-93: thread t of
-// ceil(n/4) threads takes stream values 4t..4t+3 for elements t + j*ceil(n/4)).
+// drawn as generate_random_uniform does it (random.h:63-85): its launch has
+// T = ceil(ceil(n/4) / 128) * 128 threads (common.h:289-293), and thread t takes
+// stream values 4t..4t+3 for elements t + j*T, j = 0..3; the stream then
+// advances by n.
 void Model::init_params_host(uint32_t seed, float* out) const {
     std::seed_seq seq{seed};
     std::vector<uint32_t> seeds(2);
@@ -263,7 +265,7 @@ void Model::init_params_host(uint32_t seed, float* out) const {
     for (const Segment& s : L.segs) {
         if (s.type != SEG_GRID || !s.first_use) continue;
         const size_t count = s.n_params;
-        const size_t n_threads = (count + 3) / 4;
+        const size_t n_threads = ((count + 3) / 4 + 127) / 128 * 128;
         const float lo = -1e-4f, hi = 1e-4f, span = hi - lo;
         Pcg32 r = rng;
         float* t = out + s.param_offset;

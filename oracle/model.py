@@ -450,3 +450,148 @@ def xavier_uniform(layout: Layout, seed: int, grid_range: float = 1e-4) -> np.nd
         parts.append(rng.uniform(-s, s, size=rows * cols))
     parts.append(rng.uniform(-grid_range, grid_range, size=layout.n_params - layout.n_net_params))
     return np.concatenate(parts).astype(np.float32)
+
+
+# ---------------------------------------------------------------- tcnn initialisation
+# Parameter initialisation of tiny-cuda-nn's Trainer, restated with Python
+# integers and numpy uint64 arithmetic (wrapping mod 2^64):
+#   * Trainer(seed): std::seed_seq{seed}, two words generated, pcg32 seeded
+#     with the first (include/tiny-cuda-nn/trainer.h:53-60);
+#   * network first (network_with_input_encoding.h:121-127): every weight
+#     matrix Xavier-uniform from the host stream, in order, output matrix with
+#     its padded rows (src/fully_fused_mlp.cu:863-888, gpu_matrix.h:291-306);
+#   * then each grid (composite.h:414-419, repeated_composite.h:348-353; a
+#     shared grid once): U[-1e-4, 1e-4] by generate_random_uniform
+#     (grid.h:1353-1356, random.h:63-85, common.h:289-293).
+
+_U64 = (1 << 64) - 1
+PCG_MULT = 0x5851F42D4C957F2D
+
+
+class Pcg32:
+    """PCG32, XSH-RR output of the 64-bit LCG (O'Neill 2014, pcg-random.org),
+    seeded as pcg32(initstate, initseq = 1) in tcnn's pcg32.h."""
+
+    def __init__(self, initstate: int, initseq: int = 1):
+        self.state = 0
+        self.inc = ((initseq << 1) | 1) & _U64
+        self.next_uint()
+        self.state = (self.state + initstate) & _U64
+        self.next_uint()
+
+    @staticmethod
+    def output(old: int) -> int:
+        x = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((x >> rot) | (x << ((-rot) & 31))) & 0xFFFFFFFF
+
+    def next_uint(self) -> int:
+        old = self.state
+        self.state = (old * PCG_MULT + self.inc) & _U64
+        return self.output(old)
+
+    def next_float(self) -> np.float32:
+        # (u >> 9 | 0x3f800000) read as a float, minus 1: exactly (u >> 9) / 2^23
+        return np.float32((self.next_uint() >> 9) * 2.0 ** -23)
+
+    def jump(self, delta: int):
+        """(A, C) with state_{k+delta} = A * state_k + C (mod 2^64), by repeated
+        squaring of the affine map (Brown 1994)."""
+        acc_a, acc_c, a, c = 1, 0, PCG_MULT, self.inc
+        while delta:
+            if delta & 1:
+                acc_a, acc_c = (acc_a * a) & _U64, (acc_c * a + c) & _U64
+            a, c = (a * a) & _U64, ((a + 1) * c) & _U64
+            delta >>= 1
+        return acc_a, acc_c
+
+    def advance(self, delta: int):
+        a, c = self.jump(delta)
+        self.state = (a * self.state + c) & _U64
+
+    def next_uints(self, n: int) -> np.ndarray:
+        """The next n outputs as uint32, states built by doubling jumps."""
+        states = np.empty(max(n, 1), np.uint64)
+        states[0] = self.state
+        m = 1
+        while m < n:
+            a, c = self.jump(m)
+            k = min(m, n - m)
+            states[m:m + k] = states[:k] * np.uint64(a) + np.uint64(c)
+            m *= 2
+        states = states[:n]
+        self.advance(n)
+        x = (((states >> np.uint64(18)) ^ states) >> np.uint64(27)) & np.uint64(0xFFFFFFFF)
+        rot = states >> np.uint64(59)
+        out = (x >> rot) | (x << ((np.uint64(32) - rot) & np.uint64(31)))
+        return (out & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
+def seed_seq_generate(seeds, n: int) -> list:
+    """std::seed_seq(seeds).generate over n words, as the C++ standard specifies
+    it ([rand.util.seedseq])."""
+    v = [int(s) & 0xFFFFFFFF for s in seeds]
+    s = len(v)
+    b = [0x8B8B8B8B] * n
+    t = 11 if n >= 623 else 7 if n >= 68 else 5 if n >= 39 else 3 if n >= 7 else (n - 1) // 2
+    p = (n - t) // 2
+    q = p + t
+    m = max(s + 1, n)
+    T = lambda x: x ^ (x >> 27)
+    M32 = 0xFFFFFFFF
+    for k in range(m):
+        r1 = (1664525 * T(b[k % n] ^ b[(k + p) % n] ^ b[(k - 1) % n])) & M32
+        r2 = (r1 + (s if k == 0 else (k % n + v[k - 1]) if k <= s else k % n)) & M32
+        b[(k + p) % n] = (b[(k + p) % n] + r1) & M32
+        b[(k + q) % n] = (b[(k + q) % n] + r2) & M32
+        b[k % n] = r2
+    for k in range(m, m + n):
+        r3 = (1566083941 * T((b[k % n] + b[(k + p) % n] + b[(k - 1) % n]) & M32)) & M32
+        r4 = (r3 - k % n) & M32
+        b[(k + p) % n] ^= r3
+        b[(k + q) % n] ^= r4
+        b[k % n] = r4
+    return b
+
+
+def uniform_table(rng: Pcg32, n: int, lo: float, hi: float) -> np.ndarray:
+    """generate_random_uniform<float>(rng, n, out, lo, hi): T = 128 * ceil(ceil(n/4)
+    / 128) threads, thread t draws stream values 4t..4t+3 for elements t + j*T;
+    value v -> v * (hi - lo) + lo with one rounding (the device contracts it to
+    an FMA; in float64 the product and sum are exact here, so one float32
+    rounding reproduces it).  The stream advances by n afterwards."""
+    T = ((n + 3) // 4 + 127) // 128 * 128
+    start = Pcg32.__new__(Pcg32)
+    start.state, start.inc = rng.state, rng.inc
+    u = start.next_uints(4 * T)
+    v = (u >> np.uint32(9)).astype(np.float64) * 2.0 ** -23
+    vals = v.reshape(T, 4).T.reshape(-1)[:n]
+    lo32, hi32 = np.float32(lo), np.float32(hi)
+    span = np.float64(np.float32(hi32 - lo32))
+    out = (vals * span + np.float64(lo32)).astype(np.float32)
+    rng.advance(n)
+    return out
+
+
+def tcnn_initial_params(layout: Layout, seed: int = 1337) -> np.ndarray:
+    """The float32 parameter vector tcnn's Trainer(model, ..., seed) starts from."""
+    rng = Pcg32(seed_seq_generate([seed], 2)[0])
+    n = layout.net
+    out = np.zeros(layout.n_params, np.float32)
+    shapes = [(n.width, n.n_input)] + [(n.width, n.width)] * (n.n_hidden_layers - 1) + [(n.n_output_padded, n.width)]
+    off = 0
+    two = np.float32(2.0)
+    for rows, cols in shapes:
+        scale = np.sqrt(np.float32(6.0) / np.float32(rows + cols), dtype=np.float32)
+        for k in range(rows * cols):
+            out[off + k] = rng.next_float() * two * scale - scale
+        off += rows * cols
+    grids = {}
+    for i, s in enumerate(layout.segs):
+        if s.type == SEG_GRID and s.param_offset not in grids:
+            offs, _, _ = layout.grid_levels(i)
+            grids[s.param_offset] = int(offs[-1]) * s.n_features
+    for p0 in sorted(grids):
+        cnt = grids[p0]
+        out[p0:p0 + cnt] = uniform_table(rng, cnt, -1e-4, 1e-4)
+    return out

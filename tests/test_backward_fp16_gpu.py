@@ -203,3 +203,22 @@ def test_recompute_above_threshold(cuda, precision):
     ref = (g_a[net:] + g_b[net:]).double()
     err = (g[net:].double() - ref).abs().max().item()
     assert err <= 1e-5 * ref.abs().max().item(), f"grid gradient err {err:.3e}"
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_inference_above_chunk(cuda, precision):
+    """Inference over more than 2^20 queries runs the encoding and MLP in chunks of
+    2^20 at input/output offsets; the result must equal inference over the two
+    halves bit for bit."""
+    cfg = configs.VISIBILITY
+    print(f"case inference chunks precision={precision}", flush=True)
+    m = _model(cfg, precision)
+    L = M.Layout(6, 1, cfg["encoding"], cfg["network"])
+    m.set_params(torch.from_numpy(M.xavier_uniform(L, 38, grid_range=0.5)).cuda())
+    h = RECOMPUTE_ABOVE
+    n = h + 4001
+    x = torch.from_numpy(_inputs(cfg, n, 39)).cuda()
+    out = m.inference(x)
+    out_a = m.inference(x[:h].contiguous())
+    out_b = m.inference(x[h:].contiguous())
+    assert torch.equal(out, torch.cat([out_a, out_b])), "chunked inference differs from per-half inference"
