@@ -317,6 +317,15 @@ int nlod_wavefront_run(nlod_wavefront* wavefront, void* stream, uint32_t n, cons
                        const float* d_thr_in, float* d_vis_out, float* d_thr_out);
 int nlod_wavefront_run_shape(nlod_wavefront* wavefront, void* stream, uint32_t n, const float* d_vis_in,
                              const float* d_thr_in, float* d_vis_out, float* d_thr_out, int shape);
+/* As nlod_wavefront_run with one threshold per segment, d_thresholds float[n]
+ * (neural_result.voxel_threshold, filled by the march, raymarch_impl,
+ * rptr/cuda/wavefront_neural_pt.cuh:124-136, and compared as visibility >=
+ * threshold in check_visibility, wavefront_neural_throughput_visibility_lod.cu:
+ * 320-325; nlod_lod_grid_march_thresholds gives these values).  shape -1 =
+ * the engine's choice, 0 = split launches, k > 0 = fused kernel. */
+int nlod_wavefront_run_thresholds(nlod_wavefront* wavefront, void* stream, uint32_t n, const float* d_vis_in,
+                                  const float* d_thr_in, const float* d_thresholds, float* d_vis_out,
+                                  float* d_thr_out, int shape);
 
 /* A sequence of n_batches wavefronts (counts on the host) stored back to back
  * in the buffers.  use_graph: the sequence is captured in a hipGraph on the
@@ -398,6 +407,30 @@ int nlod_lod_grid_level(const nlod_lod_grid* grid, uint32_t lod, const uint32_t*
 int nlod_lod_grid_march(const nlod_lod_grid* grid, void* stream, uint32_t lod, uint32_t n, const float* d_origins,
                         const float* d_dirs, const float* d_t_min, const uint32_t* d_prev_voxel,
                         uint32_t* d_voxel_out, float* d_t_near_out, float* d_t_far_out);
+/* Number of occupied voxels of level `lod` (the reference's
+ * gridMetaData()->activeVoxelCount(), neural_lod_learning_init.cu:272).
+ * Needs a power-of-two level resolution <= 1024.  Synchronizes the stream. */
+int nlod_lod_grid_active_voxels(const nlod_lod_grid* grid, void* stream, uint32_t lod, uint64_t* count_out);
+/* Per-voxel visibility thresholds of level `lod`: d_thresholds float[n], one
+ * per occupied voxel in increasing Morton order of (x, y, z) (x in bit 0) --
+ * the order and file layout of the reference's optimal-threshold files
+ * (read_vector_from_file, neural_lod_learning.h:482-496) -- each stored as
+ * min(max(t, min_threshold), max_threshold).  n must equal the level's
+ * occupied voxel count.  Replaces update_grid_with_thresholds
+ * (neural_lod_learning.cu:50-121).  Levels without thresholds report 0.5 for
+ * every voxel (the initial value, neural_lod_learning_init.cu:242).  Needs a
+ * power-of-two level resolution <= 1024; keeps R^3 floats on the device.
+ * Synchronizes the stream. */
+int nlod_lod_grid_set_thresholds(nlod_lod_grid* grid, void* stream, uint32_t lod, uint64_t n,
+                                 const float* d_thresholds, float min_threshold, float max_threshold);
+/* nlod_lod_grid_march that also writes the hit voxel's threshold,
+ * d_threshold_out float[n] (-1 where the ray leaves the grid), as
+ * first_active_voxel_intersection returns voxel_threshold
+ * (neural_lod_learning_common.cuh:784-832). */
+int nlod_lod_grid_march_thresholds(const nlod_lod_grid* grid, void* stream, uint32_t lod, uint32_t n,
+                                   const float* d_origins, const float* d_dirs, const float* d_t_min,
+                                   const uint32_t* d_prev_voxel, uint32_t* d_voxel_out, float* d_t_near_out,
+                                   float* d_t_far_out, float* d_threshold_out);
 
 /* ------------------------------------------ per-voxel visibility thresholds */
 

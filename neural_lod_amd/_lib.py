@@ -97,6 +97,7 @@ SIGNATURES: dict[str, list] = {
     "nlod_wavefront_max_shape": [_vp, C.POINTER(_i32)],
     "nlod_wavefront_run": [_vp, _vp, _u32, _vp, _vp, _vp, _vp],
     "nlod_wavefront_run_shape": [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _i32],
+    "nlod_wavefront_run_thresholds": [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _i32],
     "nlod_wavefront_run_batches": [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _i32],
     "nlod_labeler_create": [_vp, _vp, _u64, _u32, _u32, C.POINTER(_vp)],
     "nlod_labeler_destroy": [_vp],
@@ -107,6 +108,9 @@ SIGNATURES: dict[str, list] = {
     "nlod_lod_grid_destroy": [_vp],
     "nlod_lod_grid_level": [_vp, _u32, C.POINTER(_vp), C.POINTER(VoxInfo)],
     "nlod_lod_grid_march": [_vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "nlod_lod_grid_march_thresholds": [_vp, _vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "nlod_lod_grid_active_voxels": [_vp, _vp, _u32, C.POINTER(_u64)],
+    "nlod_lod_grid_set_thresholds": [_vp, _vp, _u32, _u64, _vp, C.c_float, C.c_float],
     "nlod_optimal_thresholds": [_vp, _u32, _u32, _vp, _vp, _u32, _i32, C.c_float, _vp, _vp],
 }
 

@@ -11,7 +11,10 @@
 // visibility MLP, then the throughput encoding and MLP, all with weights of
 // both networks resident in LDS, and writes the visibility and, for segments
 // whose visibility reaches the threshold, the throughput (0 otherwise).
-// Tiles with no visible segment skip the throughput network.
+// Tiles with no visible segment skip the throughput network.  The threshold is
+// the engine's fixed value, or per segment the threshold of the voxel the
+// segment's march stopped in (neural_result.voxel_threshold, compared as
+// visibility >= threshold in check_visibility, :320-325).
 //
 // The launch shape (blocks per CU; or the split per-level encode + MLP
 // launches, faster for large wavefronts) is chosen per wavefront-size bucket by
@@ -66,6 +69,7 @@ __device__ __forceinline__ floatx4 run_net(const NetArgs& a, const half_t* __res
 
 template <int W, int F, int MAXD>
 __global__ __launch_bounds__(kBlock) void wavefront_kernel(uint32_t N, NetArgs vis, NetArgs thr, float threshold,
+                                                           const float* __restrict__ seg_threshold,
                                                            const half_t* __restrict__ packed_vis,
                                                            const half_t* __restrict__ packed_thr,
                                                            const float* __restrict__ vis_in,
@@ -96,7 +100,8 @@ __global__ __launch_bounds__(kBlock) void wavefront_kernel(uint32_t N, NetArgs v
                 if (oi < vis.n_out) vis_out[(size_t)q * vis.n_out + oi] = act_fwd(vis.out_act, ov[i]);
             }
         }
-        const bool hit = valid && vis_n >= threshold;
+        const float t_q = seg_threshold ? seg_threshold[qc] : threshold;
+        const bool hit = valid && vis_n >= t_q;
         if (__ballot(hit) == 0) {  // no visible segment in the tile: no throughput query
             if (valid && g == 0)
                 for (uint32_t oi = 0; oi < thr.n_out; ++oi) thr_out[(size_t)q * thr.n_out + oi] = 0.0f;
@@ -113,12 +118,14 @@ __global__ __launch_bounds__(kBlock) void wavefront_kernel(uint32_t N, NetArgs v
     }
 }
 
-// Plain form: thr_out rows of segments below the threshold are zeroed.
+// Plain form: thr_out rows of segments below their threshold are zeroed.
 __global__ void gate_kernel(uint32_t N, uint32_t n_vis, uint32_t n_thr, float threshold,
-                            const float* __restrict__ vis_out, float* __restrict__ thr_out) {
+                            const float* __restrict__ seg_threshold, const float* __restrict__ vis_out,
+                            float* __restrict__ thr_out) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= N) return;
-    if (vis_out[(size_t)q * n_vis] < threshold)
+    const float t_q = seg_threshold ? seg_threshold[q] : threshold;
+    if (!(vis_out[(size_t)q * n_vis] >= t_q))
         for (uint32_t i = 0; i < n_thr; ++i) thr_out[(size_t)q * n_thr + i] = 0.0f;
 }
 
@@ -222,7 +229,7 @@ struct Wavefront {
     }
 
     void run_fused(hipStream_t s, uint32_t N, const float* vin, const float* tin, float* vout, float* tout,
-                   int blocks_per_cu) {
+                   int blocks_per_cu, const float* seg_thr = nullptr) {
         const NetArgs av = args_of(vis, 0);
         const NetArgs at = args_of(thr, av.packed_halves);
         const uint32_t tiles = div_round_up(N, 16);
@@ -230,31 +237,33 @@ struct Wavefront {
             std::max<uint32_t>(1, std::min<uint32_t>(div_round_up(tiles, kBlock / 64), (uint32_t)(cu_count() * blocks_per_cu)));
         dispatch_wf(vis.L.net.width, F, maxdeg, [&](auto Wc, auto Fc, auto Dc) {
             auto kern = wavefront_kernel<decltype(Wc)::value, decltype(Fc)::value, decltype(Dc)::value>;
-            hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), smem, s, N, av, at, threshold,
+            hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), smem, s, N, av, at, threshold, seg_thr,
                                static_cast<const half_t*>(vis.packed_fwd(true)),
                                static_cast<const half_t*>(thr.packed_fwd(true)), vin, tin, vout, tout);
         });
         NLOD_LAUNCH_CHECK();
     }
 
-    void run_split(hipStream_t s, uint32_t N, const float* vin, const float* tin, float* vout, float* tout) {
+    void run_split(hipStream_t s, uint32_t N, const float* vin, const float* tin, float* vout, float* tout,
+                   const float* seg_thr = nullptr) {
         vis.inference(s, N, vin, vout);
         thr.inference(s, N, tin, tout);
         hipLaunchKernelGGL(gate_kernel, dim3(div_round_up(N, 256)), dim3(256), 0, s, N, vis.L.net.n_output,
-                           thr.L.net.n_output, threshold, vout, tout);
+                           thr.L.net.n_output, threshold, seg_thr, vout, tout);
         NLOD_LAUNCH_CHECK();
     }
 
     // shape < 0: the tuned choice for N; 0: split launches; k > 0: fused kernel, k blocks per CU
-    void run(hipStream_t s, uint32_t N, const float* vin, const float* tin, float* vout, float* tout, int shape = -1) {
+    void run(hipStream_t s, uint32_t N, const float* vin, const float* tin, float* vout, float* tout, int shape = -1,
+             const float* seg_thr = nullptr) {
         if (N == 0) return;
         vis.require_params(true);
         thr.require_params(true);
         const int c = shape < 0 ? choice[bucket_of(N)] : shape;
         if (c > 0 && !(fp16 && fused_ok)) fail(NLOD_ERR_UNSUPPORTED, "the fused wavefront kernel does not apply here");
         if (c > max_blocks_per_cu) fail(NLOD_ERR_INVALID_ARGUMENT, "more blocks per CU than fit");
-        if (c > 0) run_fused(s, N, vin, tin, vout, tout, c);
-        else run_split(s, N, vin, tin, vout, tout);
+        if (c > 0) run_fused(s, N, vin, tin, vout, tout, c, seg_thr);
+        else run_split(s, N, vin, tin, vout, tout, seg_thr);
     }
 
     // Times each candidate once per bucket (synchronizes the stream).
@@ -419,6 +428,17 @@ int nlod_wavefront_run_shape(nlod_wavefront* wf, void* stream, uint32_t n, const
         NLOD_CHECK(wf && (n == 0 || (d_vis_in && d_thr_in && d_vis_out && d_thr_out)), "null argument");
         NLOD_CHECK(shape >= 0, "shape must be 0 (split) or a block count per CU");
         wf->w->run(nlod::as_stream(stream), n, d_vis_in, d_thr_in, d_vis_out, d_thr_out, shape);
+    });
+}
+
+int nlod_wavefront_run_thresholds(nlod_wavefront* wf, void* stream, uint32_t n, const float* d_vis_in,
+                                  const float* d_thr_in, const float* d_thresholds, float* d_vis_out,
+                                  float* d_thr_out, int shape) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(wf && (n == 0 || (d_vis_in && d_thr_in && d_thresholds && d_vis_out && d_thr_out)),
+                   "null argument");
+        NLOD_CHECK(shape >= -1, "shape must be -1 (engine's choice), 0 (split) or a block count per CU");
+        wf->w->run(nlod::as_stream(stream), n, d_vis_in, d_thr_in, d_vis_out, d_thr_out, shape, d_thresholds);
     });
 }
 

@@ -54,10 +54,32 @@ class LodGrid:
 
         return torch.as_tensor(_View(), device=self.device).clone()
 
+    def active_voxels(self, lod: int) -> int:
+        """Occupied voxels of level `lod` (the length of its threshold file)."""
+        c = C.c_uint64()
+        _lib.call("nlod_lod_grid_active_voxels", self._h, C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream),
+                  int(lod), C.byref(c))
+        return int(c.value)
+
+    def set_thresholds(self, lod: int, thresholds: torch.Tensor, min_threshold: float = 0.1,
+                       max_threshold: float = 0.9) -> None:
+        """Per-voxel thresholds of level `lod`, one per occupied voxel in
+        increasing Morton order (the layout of the reference's
+        *_lod_<l>_optimal_thresholds.bin files), clamped to [min, max]
+        (defaults: threshold_min_value / threshold_max_value,
+        rptr/cuda/neural-lod/neural_lod_learning.h:223-224)."""
+        t = thresholds
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()
+                and t.dim() == 1):
+            raise TypeError("thresholds must be a contiguous 1-D float32 CUDA tensor")
+        _lib.call("nlod_lod_grid_set_thresholds", self._h, C.c_void_p(torch.cuda.current_stream(t.device).cuda_stream),
+                  int(lod), t.numel(), C.c_void_p(t.data_ptr()), float(min_threshold), float(max_threshold))
+
     def march(self, lod: int, origins: torch.Tensor, dirs: torch.Tensor, t_min: torch.Tensor | None = None,
-              prev: torch.Tensor | None = None):
+              prev: torch.Tensor | None = None, thresholds: bool = False):
         """First occupied voxel along each ray: (voxel int64 [n] with -1 for
-        none, t_near [n], t_far [n])."""
+        none, t_near [n], t_far [n]), plus the voxel's threshold [n] (-1 for
+        none) when `thresholds`."""
         n = origins.shape[0]
         for t, name in ((origins, "origins"), (dirs, "dirs")):
             if t.shape != (n, 3) or t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
@@ -66,11 +88,17 @@ class LodGrid:
         tn = torch.empty(n, dtype=torch.float32, device=origins.device)
         tf = torch.empty(n, dtype=torch.float32, device=origins.device)
         stream = C.c_void_p(torch.cuda.current_stream(origins.device).cuda_stream)
-        _lib.call("nlod_lod_grid_march", self._h, stream, int(lod), n, C.c_void_p(origins.data_ptr()),
-                  C.c_void_p(dirs.data_ptr()), C.c_void_p(t_min.data_ptr()) if t_min is not None else None,
-                  C.c_void_p(prev.data_ptr()) if prev is not None else None, C.c_void_p(vox.data_ptr()),
-                  C.c_void_p(tn.data_ptr()), C.c_void_p(tf.data_ptr()))
+        args = (self._h, stream, int(lod), n, C.c_void_p(origins.data_ptr()), C.c_void_p(dirs.data_ptr()),
+                C.c_void_p(t_min.data_ptr()) if t_min is not None else None,
+                C.c_void_p(prev.data_ptr()) if prev is not None else None, C.c_void_p(vox.data_ptr()),
+                C.c_void_p(tn.data_ptr()), C.c_void_p(tf.data_ptr()))
+        thr = None
+        if thresholds:
+            thr = torch.empty(n, dtype=torch.float32, device=origins.device)
+            _lib.call("nlod_lod_grid_march_thresholds", *args, C.c_void_p(thr.data_ptr()))
+        else:
+            _lib.call("nlod_lod_grid_march", *args)
         v = vox.to(torch.int64) & 0xFFFFFFFF
         v[v == NO_VOXEL] = -1
-        return v, tn, tf
+        return (v, tn, tf, thr) if thresholds else (v, tn, tf)
 

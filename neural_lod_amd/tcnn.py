@@ -385,12 +385,19 @@ class Wavefront:
         return (torch.empty((n, self.visibility.n_output_dims), dtype=torch.float32, device=device),
                 torch.empty((n, self.throughput.n_output_dims), dtype=torch.float32, device=device))
 
-    def run(self, vis_in: torch.Tensor, thr_in: torch.Tensor, shape: int | None = None, out=None):
+    def run(self, vis_in: torch.Tensor, thr_in: torch.Tensor, shape: int | None = None, out=None,
+            thresholds: torch.Tensor | None = None):
+        """thresholds: optional float32 [n], each segment's voxel threshold
+        (LodGrid.march(..., thresholds=True)); default the engine's value."""
         n = vis_in.shape[0]
         _check_f32(vis_in, "vis_in", (n, self.visibility.n_input_dims))
         _check_f32(thr_in, "thr_in", (n, self.throughput.n_input_dims))
         vo, to = out if out is not None else self._outputs(n, vis_in.device)
-        if shape is None:
+        if thresholds is not None:
+            _check_f32(thresholds, "thresholds", (n,))
+            _lib.call("nlod_wavefront_run_thresholds", self._h, _stream(vis_in.device), n, _ptr(vis_in), _ptr(thr_in),
+                      _ptr(thresholds), _ptr(vo), _ptr(to), -1 if shape is None else int(shape))
+        elif shape is None:
             _lib.call("nlod_wavefront_run", self._h, _stream(vis_in.device), n, _ptr(vis_in), _ptr(thr_in), _ptr(vo),
                       _ptr(to))
         else:

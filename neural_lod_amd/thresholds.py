@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -32,3 +33,28 @@ def optimal_thresholds(labels: torch.Tensor, probas: torch.Tensor, n_voxels: int
               C.c_void_p(probas.data_ptr()), n_thresholds, m, float(beta), C.c_void_p(thr.data_ptr()),
               C.c_void_p(sc.data_ptr()) if sc is not None else None)
     return (thr, sc) if return_scores else thr
+
+
+def threshold_path(weights_directory: str, weights_filename_prefix: str, lod: int) -> str:
+    """<weights dir>../optimal_thresholds/<prefix>_lod_<l>_optimal_thresholds.bin
+    (get_optimal_threshold_path, rptr/cuda/neural-lod/neural_lod_learning.h:332-334)."""
+    return f"{weights_directory}../optimal_thresholds/{weights_filename_prefix}_lod_{int(lod)}_optimal_thresholds.bin"
+
+
+def save_thresholds(path, thresholds) -> None:
+    """Raw little-endian float32 values, nothing else (write_vector_to_file,
+    neural_lod_learning.h:464-480)."""
+    a = thresholds.detach().cpu().numpy() if isinstance(thresholds, torch.Tensor) else np.asarray(thresholds)
+    np.ascontiguousarray(a, dtype="<f4").tofile(path)
+
+
+def load_thresholds(path, n: int) -> np.ndarray:
+    """The first n float32 values of a threshold file (read_vector_from_file,
+    neural_lod_learning.h:482-496, which reads n = the level's occupied voxel
+    count).  A file holding fewer than n values is an error here (the
+    reference leaves the missing values zero)."""
+    a = np.fromfile(path, dtype="<f4", count=int(n))
+    if a.size != int(n):
+        raise ValueError(f"{path}: {a.size} thresholds, expected {int(n)}")
+    return a.astype(np.float32)
+

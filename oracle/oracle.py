@@ -174,6 +174,42 @@ def march(table: np.ndarray, res: int, bmin, unit, origins: np.ndarray, dirs: np
     return vox, tn, tf
 
 
+def morton3(x, y, z) -> np.ndarray:
+    """encode_morton3 (rptr/cuda/neural-lod/neural_lod_learning.h:519-532): the
+    low 10 bits of x, y, z interleaved, x in bit 0, y in bit 1, z in bit 2."""
+    def part(v):
+        v = np.asarray(v, np.uint64) & np.uint64(0x3FF)
+        v = (v ^ (v << np.uint64(16))) & np.uint64(0xFF0000FF)
+        v = (v ^ (v << np.uint64(8))) & np.uint64(0x0300F00F)
+        v = (v ^ (v << np.uint64(4))) & np.uint64(0x030C30C3)
+        v = (v ^ (v << np.uint64(2))) & np.uint64(0x09249249)
+        return v
+    return ((part(z) << np.uint64(2)) + (part(y) << np.uint64(1)) + part(x)).astype(np.uint32)
+
+
+def active_voxels(table: np.ndarray, res: int) -> np.ndarray:
+    """Linear indices x + y R + z R^2 of the occupied voxels of a bit table."""
+    bits = unpack_bits(np.ascontiguousarray(table, np.uint32), res).reshape(-1)
+    return np.flatnonzero(bits)
+
+
+def voxel_thresholds(table: np.ndarray, res: int, thresholds: np.ndarray, lo: float, hi: float) -> np.ndarray:
+    """Dense float32 [R^3] thresholds as update_grid_with_thresholds leaves the
+    level (neural_lod_learning.cu:50-121): the j-th occupied voxel in
+    increasing Morton order (the sorted morton_sparse_voxel_idx_mappings,
+    neural_lod_learning_init.cu:288-295) takes glm::clamp(t[j], lo, hi) =
+    min(max(t[j], lo), hi); every other voxel keeps 0.5."""
+    lin = active_voxels(table, res)
+    if lin.size != len(thresholds):
+        raise ValueError("one threshold per occupied voxel")
+    x, y, z = lin % res, (lin // res) % res, lin // (res * res)
+    order = np.argsort(morton3(x, y, z), kind="stable")
+    t = np.asarray(thresholds, np.float32)
+    dense = np.full(res ** 3, np.float32(0.5), np.float32)
+    dense[lin[order]] = np.minimum(np.maximum(t, np.float32(lo)), np.float32(hi))
+    return dense
+
+
 def optimal_thresholds(labels: np.ndarray, probas: np.ndarray, n_voxels: int, n_thresholds: int = 101,
                        metric: int = 2, beta: float = 1.0):
     """(thresholds [V], scores [V][T]) of the per-voxel threshold optimisation."""

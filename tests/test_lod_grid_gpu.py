@@ -53,3 +53,58 @@ def test_invalid_arguments(cuda):
     g = LodGrid(table, info, 2)
     with pytest.raises(NlodError):
         g.march(2, torch.zeros(4, 3, device="cuda"), torch.ones(4, 3, device="cuda"))
+
+
+@pytest.mark.parametrize("res,n_lods", [(64, 3), (128, 2)])
+def test_voxel_thresholds_and_march(oracle, cuda, res, n_lods):
+    """Per-voxel thresholds (one per occupied voxel in Morton order, clamped)
+    reported by the march for the hit voxel: equal to the oracle's dense table
+    at the oracle's voxel; 0.5 on levels without thresholds; -1 on a miss."""
+    from neural_lod_amd import voxelizer as V
+    from neural_lod_amd._lib import NlodError
+    from neural_lod_amd.lod_grid import LodGrid
+    tris = S.blob_mesh(20_000, seed=3)
+    print(f"case voxel thresholds res={res} n_lods={n_lods}", flush=True)
+    table, info = V.voxelize(torch.from_numpy(tris).cuda(), res)
+    grid = LodGrid(table, info, n_lods)
+    rng = np.random.default_rng(11)
+    level = table.cpu().numpy().view(np.uint32)
+    n = 20000
+    for lod in range(n_lods):
+        r = res >> lod
+        if lod > 0:
+            level = oracle.downsample(level, r * 2)
+        count = grid.active_voxels(lod)
+        assert count == oracle.active_voxels(level, r).size
+        li = grid.levels[lod]
+        bmin, unit = np.array(li.bbox_min), np.array(li.unit)
+        o = (bmin + rng.uniform(-0.3, r + 0.3, size=(n, 3)) * unit).astype(np.float32)
+        d = rng.normal(size=(n, 3))
+        d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+        od, dd = torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda()
+        ev, _, _ = oracle.march(level, r, bmin, unit, o, d)
+        hit = ev != 0xFFFFFFFF
+        if lod == n_lods - 1:  # this level keeps no thresholds: 0.5 everywhere
+            _, _, _, th = grid.march(lod, od, dd, thresholds=True)
+            assert np.array_equal(th.cpu().numpy(), np.where(hit, 0.5, -1.0).astype(np.float32))
+            continue
+        t = rng.uniform(-0.1, 1.1, count).astype(np.float32)
+        with pytest.raises(NlodError, match="one threshold per active voxel"):
+            grid.set_thresholds(lod, torch.from_numpy(t[:-1]).cuda())
+        grid.set_thresholds(lod, torch.from_numpy(t).cuda(), 0.1, 0.9)
+        dense = oracle.voxel_thresholds(level, r, t, 0.1, 0.9)
+        v, tn, tf, th = grid.march(lod, od, dd, thresholds=True)
+        exp = np.where(hit, dense[np.where(hit, ev, 0).astype(np.int64)], -1.0).astype(np.float32)
+        assert np.array_equal(v.cpu().numpy(), np.where(hit, ev.astype(np.int64), -1))
+        assert np.array_equal(th.cpu().numpy(), exp), f"lod {lod}: thresholds differ"
+        assert hit.mean() > 0.05 and len(np.unique(exp[hit])) > 10
+
+
+def test_thresholds_need_power_of_two_level(cuda):
+    from neural_lod_amd import voxelizer as V
+    from neural_lod_amd._lib import NlodError
+    from neural_lod_amd.lod_grid import LodGrid
+    table, info = V.voxelize(torch.from_numpy(S.blob_mesh(100, seed=1)).cuda(), 24)
+    g = LodGrid(table, info, 1)
+    with pytest.raises(NlodError, match="power-of-two"):
+        g.active_voxels(0)

@@ -1,7 +1,10 @@
 """The CPU restatement of the per-voxel threshold optimisation, pinned by a
 separable case with a known plateau and by numpy confusion counts with the
-textbook score definitions."""
+textbook score definitions; the Morton-ordered per-voxel threshold layout,
+pinned by bit-by-bit interleaving and a brute-force walk; the reference's
+shipped threshold files (format and value grid)."""
 import numpy as np
+import pytest
 
 
 def test_separable_voxel_picks_plateau_centre(oracle):
@@ -46,3 +49,72 @@ def test_scores_match_numpy_confusion_counts(oracle):
         best = np.argmax(sc, axis=1)
         assert np.all(np.abs(thr - best / (T - 1)) <= 0.5 + 1e-6)  # within the plateau after the maximum
         assert np.all(sc[np.arange(V), np.round(thr * (T - 1)).astype(int)] >= sc.max(axis=1) - 1e-4)
+
+
+# ---------------------------------------------------------------- per-voxel thresholds in the grid
+GOLDEN = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+
+
+def _interleave(x, y, z):
+    code = 0
+    for b in range(10):
+        code |= ((x >> b) & 1) << (3 * b) | ((y >> b) & 1) << (3 * b + 1) | ((z >> b) & 1) << (3 * b + 2)
+    return code
+
+
+def test_morton3_is_bit_interleave(oracle):
+    rng = np.random.default_rng(4)
+    xyz = rng.integers(0, 1024, size=(2000, 3))
+    got = oracle.morton3(xyz[:, 0], xyz[:, 1], xyz[:, 2])
+    assert [int(v) for v in got] == [_interleave(*map(int, p)) for p in xyz]
+    assert [int(oracle.morton3(*p)) for p in ((1, 0, 0), (0, 1, 0), (0, 0, 1), (1023, 1023, 1023))] == \
+        [1, 2, 4, (1 << 30) - 1]
+
+
+@pytest.mark.parametrize("res", [4, 8, 16])
+def test_voxel_thresholds_brute_force(oracle, res):
+    """Dense thresholds against a pure-Python walk over every voxel: occupied
+    voxels sorted by their interleaved code take the clamped values in order."""
+    rng = np.random.default_rng(res)
+    occ = rng.random(res ** 3) < 0.3
+    table = np.packbits(np.concatenate([occ, np.zeros(-len(occ) % 32, bool)])).view(">u4").astype(np.uint32)
+    n = int(occ.sum())
+    t = rng.uniform(-0.2, 1.2, n).astype(np.float32)
+    dense = oracle.voxel_thresholds(table, res, t, 0.1, 0.9)
+    voxels = sorted((_interleave(i % res, (i // res) % res, i // (res * res)), i) for i in range(res ** 3) if occ[i])
+    exp = np.full(res ** 3, 0.5, np.float32)
+    for j, (_, i) in enumerate(voxels):
+        exp[i] = min(max(float(t[j]), float(np.float32(0.1))), float(np.float32(0.9)))
+    assert np.array_equal(dense, exp)
+    with pytest.raises(ValueError):
+        oracle.voxel_thresholds(table, res, t[:-1], 0.1, 0.9)
+
+
+@pytest.mark.parametrize("lod,n", [(5, 683), (6, 146)])
+def test_reference_threshold_files(oracle, tmp_path, lod, n):
+    """The reference's shipped threshold files: raw float32, one value per
+    occupied voxel; every value is t / (T - 1) in float32 for T = 101, the
+    default threshold count of the oracle and the kernel -- a formula that a
+    plausible slip (T instead of T - 1) would not reproduce.  Saving what was
+    loaded gives the same bytes."""
+    from neural_lod_amd.thresholds import load_thresholds, save_thresholds, threshold_path
+    src = GOLDEN / f"arcade_world_final_lod_{lod}_optimal_thresholds.bin"
+    a = load_thresholds(src, n)
+    assert a.shape == (n,) and a.dtype == np.float32
+    with pytest.raises(ValueError):
+        load_thresholds(src, n + 1)
+
+    def grid(T):
+        return set((np.arange(T, dtype=np.float32) / np.float32(T - 1)).tolist())
+
+    assert set(a.tolist()) <= grid(101)
+    assert not set(a.tolist()) <= grid(100) and not set(a.tolist()) <= grid(102)
+    # the oracle's argmax -> threshold mapping lands on the same float values
+    probas = np.linspace(0, 1, 101, dtype=np.float32)
+    thr, _ = oracle.optimal_thresholds((probas >= 0.37).astype(np.float32), probas, 1)
+    assert set(thr.tolist()) <= grid(101)
+    out = tmp_path / "t.bin"
+    save_thresholds(out, a)
+    assert out.read_bytes() == src.read_bytes()
+    assert threshold_path("w/", "arcade_world_final", lod).endswith(
+        f"w/../optimal_thresholds/arcade_world_final_lod_{lod}_optimal_thresholds.bin")

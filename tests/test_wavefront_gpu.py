@@ -110,3 +110,31 @@ def test_autotuner(cuda):
         assert 0 <= c <= wf.max_shape()
     got = wf.run(vi, ti)
     assert torch.equal(got[0], ref[0]) and torch.equal(got[1], ref[1])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_per_segment_thresholds(cuda, precision):
+    """One threshold per segment (each segment's voxel threshold): the throughput
+    is kept where visibility >= that segment's threshold and 0 elsewhere, bit for
+    bit equal to the unthresholded throughput; constant thresholds equal the
+    engine's own value; every launch shape agrees."""
+    from neural_lod_amd.tcnn import Wavefront
+    wf, nets = _pair(precision)
+    n = 5003
+    print(f"case per-segment thresholds precision={precision} n={n}", flush=True)
+    vi, ti = _inputs(n, 8)
+    open_wf = Wavefront(nets[0][0], nets[1][0], -1.0)  # every segment visible
+    v_all, t_all = open_wf.run(vi, ti)
+    g = torch.Generator().manual_seed(9)
+    thr = torch.rand(n, generator=g).cuda()
+    thr[::7] = v_all[::7, 0]  # ties: visibility == threshold counts as visible
+    shapes = [None, 0] + (list(range(1, wf.max_shape() + 1)) if precision == 1 else [])
+    for shape in shapes:
+        vo, to = wf.run(vi, ti, shape=shape, thresholds=thr)
+        assert torch.equal(vo, v_all), f"visibility differs at shape {shape}"
+        keep = (vo[:, 0] >= thr)[:, None]
+        assert torch.equal(to, torch.where(keep, t_all, torch.zeros_like(t_all))), f"shape {shape}"
+        assert 0.05 < keep.float().mean().item() < 0.95
+        vc, tc = wf.run(vi, ti, shape=shape, thresholds=torch.full((n,), THRESHOLD, device="cuda"))
+        vs, ts = wf.run(vi, ti, shape=shape)
+        assert torch.equal(vc, vs) and torch.equal(tc, ts), f"constant thresholds differ at shape {shape}"
