@@ -33,7 +33,7 @@ class GridBackward {
     struct HostPlan;
     HostPlan* host_ = nullptr;
     DevBuf d_rows_, d_bins_;
-    DevBuf counts_, totals_, bin_start_, chunk_start_, meta_, keys_, records_, partial_;
+    DevBuf counts_, totals_, bin_start_, chunk_start_, meta_, records_, partial_;
 };
 
 }  // namespace nlod

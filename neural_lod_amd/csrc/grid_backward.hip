@@ -10,7 +10,9 @@
 //   1. count    per (query block, table tile) how many records fall in the
 //               tile; also max |dL/dy| of the pass (sets the fixed-point scale)
 //   2. scan     tile-major exclusive scan -> every block's slot range per tile
-//   3. scatter  records (entry within the tile; F contributions) to their slots
+//   3. scatter  records (entry within the tile; F contributions) to their slots,
+//               each record one aligned vector of next_pow2(F + 1) words, so a
+//               record is one store (and one cache line) and one load
 //   4. reduce   one workgroup per group of <= kGroupRecs records of one tile:
 //               contributions as 64-bit fixed point (2^S units, S from the
 //               max so that no sum can overflow) added with ds_add_u64; a tile
@@ -226,7 +228,43 @@ __global__ __launch_bounds__(kBlock) void scan_bins_kernel(uint32_t n_bins, uint
     }
 }
 
-// Pass 3: records: keys[slot] = entry within the tile, vals[slot][F].
+// Words per record: the key and F values, padded to a power of two.
+template <int F>
+struct RecWords {
+    static constexpr int value = F + 1 <= 2 ? 2 : F + 1 <= 4 ? 4 : F + 1 <= 8 ? 8 : 16;
+};
+
+template <int RW>
+__device__ __forceinline__ void store_rec(uint32_t* __restrict__ dst, const uint32_t (&w)[RW]) {
+    if constexpr (RW == 2) {
+        *reinterpret_cast<uint2*>(dst) = make_uint2(w[0], w[1]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < RW; i += 4)
+            *reinterpret_cast<uint4*>(dst + i) = make_uint4(w[i], w[i + 1], w[i + 2], w[i + 3]);
+    }
+}
+
+template <int RW>
+__device__ __forceinline__ void load_rec(const uint32_t* __restrict__ src, uint32_t (&w)[RW]) {
+    if constexpr (RW == 2) {
+        const uint2 a = *reinterpret_cast<const uint2*>(src);
+        w[0] = a.x;
+        w[1] = a.y;
+    } else {
+#pragma unroll
+        for (int i = 0; i < RW; i += 4) {
+            const uint4 a = *reinterpret_cast<const uint4*>(src + i);
+            w[i] = a.x;
+            w[i + 1] = a.y;
+            w[i + 2] = a.z;
+            w[i + 3] = a.w;
+        }
+    }
+}
+
+// Pass 3: record of slot i = words [i * RW, (i + 1) * RW): entry within the
+// tile, then the F contributions.
 template <int F>
 __global__ __launch_bounds__(kBlock) void scatter_kernel(uint32_t N, uint32_t ld, uint32_t n_qblocks, uint32_t in_stride,
                                                          const GridBwdRow* __restrict__ rows,
@@ -234,8 +272,9 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(uint32_t N, uint32_t ld
                                                          const float* __restrict__ d_enc,
                                                          const uint32_t* __restrict__ row_offsets,
                                                          const uint32_t* __restrict__ bin_start,
-                                                         uint32_t* __restrict__ keys, float* __restrict__ vals) {
+                                                         uint32_t* __restrict__ recs) {
     constexpr uint32_t E = kTileFloats / F;
+    constexpr int RW = RecWords<F>::value;
     __shared__ uint32_t cursor[kMaxTilesPerRow];
     const GridBwdRow r = rows[blockIdx.y];
     for (uint32_t t = threadIdx.x; t < r.n_tiles; t += kBlock) {
@@ -252,9 +291,13 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(uint32_t N, uint32_t ld
         const float* x = in + (size_t)q * in_stride + r.in_offset;
         corners(r, x, [&](uint32_t idx, float w) {
             const uint32_t slot = atomicAdd(&cursor[idx / E], 1u);
-            keys[slot] = idx % E;
+            uint32_t rec[RW];
+            rec[0] = idx % E;
 #pragma unroll
-            for (int f = 0; f < F; ++f) vals[(size_t)slot * F + f] = w * g[f];
+            for (int f = 0; f < F; ++f) rec[1 + f] = __float_as_uint(w * g[f]);
+#pragma unroll
+            for (int i = F + 1; i < RW; ++i) rec[i] = 0u;
+            store_rec<RW>(recs + (size_t)slot * RW, rec);
         });
     }
 }
@@ -276,10 +319,10 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(uint32_t n_bins, const u
                                                         const uint32_t* __restrict__ bin_start,
                                                         const uint32_t* __restrict__ group_start,
                                                         const GridBwdBin* __restrict__ bins,
-                                                        const uint32_t* __restrict__ keys,
-                                                        const float* __restrict__ vals,
+                                                        const uint32_t* __restrict__ recs,
                                                         long long* __restrict__ partial, float* __restrict__ grads,
                                                         int overwrite) {
+    constexpr int RW = RecWords<F>::value;
     __shared__ unsigned long long acc[kTileFloats];
     const uint32_t gi = blockIdx.x;
     if (gi >= meta[0]) return;
@@ -304,9 +347,11 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(uint32_t n_bins, const u
 #pragma unroll
         for (uint32_t k = 0; k < kInFlight; ++k) {
             const uint32_t i = min(i0 + k * kBlock + threadIdx.x, end - 1);
-            key[k] = keys[i];
+            uint32_t rec[RW];
+            load_rec<RW>(recs + (size_t)i * RW, rec);
+            key[k] = rec[0];
 #pragma unroll
-            for (int f = 0; f < F; ++f) v[k][f] = vals[(size_t)i * F + f];
+            for (int f = 0; f < F; ++f) v[k][f] = __uint_as_float(rec[1 + f]);
         }
         // branch-free: lanes past the end add 0 to a valid entry, so every load
         // of the batch is issued before the first LDS add needs its data
@@ -478,8 +523,8 @@ void GridBackward::prepare(uint32_t N) {
     bin_start_.ensure(sizeof(uint32_t) * n_bins_);
     chunk_start_.ensure(sizeof(uint32_t) * (n_bins_ + 1));
     meta_.ensure(sizeof(uint32_t) * 4);
-    keys_.ensure(sizeof(uint32_t) * (size_t)max_records);
-    records_.ensure(sizeof(float) * (size_t)max_records * F_);
+    const size_t rec_words = F_ + 1 <= 2 ? 2 : F_ + 1 <= 4 ? 4 : F_ + 1 <= 8 ? 8 : 16;
+    records_.ensure(sizeof(uint32_t) * (size_t)max_records * rec_words);
     partial_.ensure(sizeof(long long) * (size_t)max_groups(n) * kTileFloats);
 }
 
@@ -525,12 +570,12 @@ void GridBackward::run(hipStream_t s, const float* in, uint32_t N, const float* 
             constexpr int F = decltype(Fc)::value;
             hipLaunchKernelGGL(scatter_kernel<F>, dim3(nqb, n_rows_), dim3(kBlock), 0, s, n, N, nqb, n_input_dims_,
                                d_rows_.as<GridBwdRow>(), in_c, denc_c, counts_.as<uint32_t>(),
-                               bin_start_.as<uint32_t>(), keys_.as<uint32_t>(), records_.as<float>());
+                               bin_start_.as<uint32_t>(), records_.as<uint32_t>());
             NLOD_LAUNCH_CHECK();
             hipLaunchKernelGGL(reduce_kernel<F>, dim3((uint32_t)n_groups), dim3(kBlock), 0, s, n_bins_,
                                meta_.as<uint32_t>(), totals_.as<uint32_t>(), bin_start_.as<uint32_t>(),
-                               chunk_start_.as<uint32_t>(), d_bins_.as<GridBwdBin>(), keys_.as<uint32_t>(),
-                               records_.as<float>(), partial_.as<long long>(), grads, ow);
+                               chunk_start_.as<uint32_t>(), d_bins_.as<GridBwdBin>(), records_.as<uint32_t>(),
+                               partial_.as<long long>(), grads, ow);
         });
         NLOD_LAUNCH_CHECK();
         hipLaunchKernelGGL(apply_kernel, dim3(n_bins_, kTileFloats / kBlock), dim3(kBlock), 0, s,
