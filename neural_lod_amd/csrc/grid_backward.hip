@@ -11,8 +11,9 @@
 //               tile; also max |dL/dy| of the pass (sets the fixed-point scale)
 //   2. scan     tile-major exclusive scan -> every block's slot range per tile
 //   3. scatter  records (entry within the tile; F contributions) to their slots,
-//               each record one aligned vector of next_pow2(F + 1) words, so a
-//               record is one store (and one cache line) and one load
+//               each record one vector of F + 1 words for F = 2 (12 bytes,
+//               unpadded: the reduce streams them) and of next_pow2(F + 1)
+//               words otherwise, so a record is one store and one load
 //   4. reduce   one workgroup per group of <= kGroupRecs records of one tile:
 //               contributions as 64-bit fixed point (2^S units, S from the
 //               max so that no sum can overflow) added with ds_add_u64; a tile
@@ -228,16 +229,19 @@ __global__ __launch_bounds__(kBlock) void scan_bins_kernel(uint32_t n_bins, uint
     }
 }
 
-// Words per record: the key and F values, padded to a power of two.
+// Words per record: the key and F values; padded to a power of two except for
+// F = 2, whose 3-word records load and store as one dwordx3.
 template <int F>
 struct RecWords {
-    static constexpr int value = F + 1 <= 2 ? 2 : F + 1 <= 4 ? 4 : F + 1 <= 8 ? 8 : 16;
+    static constexpr int value = F == 2 ? 3 : F + 1 <= 2 ? 2 : F + 1 <= 4 ? 4 : F + 1 <= 8 ? 8 : 16;
 };
 
 template <int RW>
 __device__ __forceinline__ void store_rec(uint32_t* __restrict__ dst, const uint32_t (&w)[RW]) {
     if constexpr (RW == 2) {
         *reinterpret_cast<uint2*>(dst) = make_uint2(w[0], w[1]);
+    } else if constexpr (RW == 3) {
+        *reinterpret_cast<uint3*>(dst) = make_uint3(w[0], w[1], w[2]);
     } else {
 #pragma unroll
         for (int i = 0; i < RW; i += 4)
@@ -251,6 +255,11 @@ __device__ __forceinline__ void load_rec(const uint32_t* __restrict__ src, uint3
         const uint2 a = *reinterpret_cast<const uint2*>(src);
         w[0] = a.x;
         w[1] = a.y;
+    } else if constexpr (RW == 3) {
+        const uint3 a = *reinterpret_cast<const uint3*>(src);
+        w[0] = a.x;
+        w[1] = a.y;
+        w[2] = a.z;
     } else {
 #pragma unroll
         for (int i = 0; i < RW; i += 4) {
@@ -523,7 +532,7 @@ void GridBackward::prepare(uint32_t N) {
     bin_start_.ensure(sizeof(uint32_t) * n_bins_);
     chunk_start_.ensure(sizeof(uint32_t) * (n_bins_ + 1));
     meta_.ensure(sizeof(uint32_t) * 4);
-    const size_t rec_words = F_ + 1 <= 2 ? 2 : F_ + 1 <= 4 ? 4 : F_ + 1 <= 8 ? 8 : 16;
+    const size_t rec_words = F_ == 2 ? 3 : F_ + 1 <= 2 ? 2 : F_ + 1 <= 4 ? 4 : F_ + 1 <= 8 ? 8 : 16;
     records_.ensure(sizeof(uint32_t) * (size_t)max_records * rec_words);
     partial_.ensure(sizeof(long long) * (size_t)max_groups(n) * kTileFloats);
 }
