@@ -350,29 +350,33 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(uint32_t n_bins, const u
     const int S = finite ? fixed_shift(mbits) : 0;
     for (uint32_t i = threadIdx.x; i < kTileFloats; i += kBlock) acc[i] = 0ull;
     __syncthreads();
-    for (uint32_t i0 = begin; i0 < end; i0 += kBlock * kInFlight) {
-        uint32_t key[kInFlight];
-        float v[kInFlight][F];
+    // software-pipelined: the next batch's loads are issued before this
+    // batch's LDS adds (lanes past the end load the last record and add 0)
+    uint32_t cur[kInFlight][RW], nxt[kInFlight][RW];
+    auto load_batch = [&](uint32_t i0, uint32_t (&dst)[kInFlight][RW]) {
 #pragma unroll
         for (uint32_t k = 0; k < kInFlight; ++k) {
             const uint32_t i = min(i0 + k * kBlock + threadIdx.x, end - 1);
-            uint32_t rec[RW];
-            load_rec<RW>(recs + (size_t)i * RW, rec);
-            key[k] = rec[0];
-#pragma unroll
-            for (int f = 0; f < F; ++f) v[k][f] = __uint_as_float(rec[1 + f]);
+            load_rec<RW>(recs + (size_t)i * RW, dst[k]);
         }
-        // branch-free: lanes past the end add 0 to a valid entry, so every load
-        // of the batch is issued before the first LDS add needs its data
+    };
+    if (begin < end) load_batch(begin, cur);
+    for (uint32_t i0 = begin; i0 < end; i0 += kBlock * kInFlight) {
+        const uint32_t i1 = i0 + kBlock * kInFlight;
+        if (i1 < end) load_batch(i1, nxt);
 #pragma unroll
         for (uint32_t k = 0; k < kInFlight; ++k) {
             const bool valid = i0 + k * kBlock + threadIdx.x < end;
 #pragma unroll
             for (int f = 0; f < F; ++f) {
-                const long long q = valid ? __float2ll_rn(ldexpf(v[k][f], S)) : 0ll;
-                atomicAdd(&acc[key[k] * F + f], (unsigned long long)q);
+                const long long q = valid ? __float2ll_rn(ldexpf(__uint_as_float(cur[k][1 + f]), S)) : 0ll;
+                atomicAdd(&acc[cur[k][0] * F + f], (unsigned long long)q);
             }
         }
+#pragma unroll
+        for (uint32_t k = 0; k < kInFlight; ++k)
+#pragma unroll
+            for (int j = 0; j < RW; ++j) cur[k][j] = nxt[k][j];
     }
     __syncthreads();
     const GridBwdBin bin = bins[b];
