@@ -124,15 +124,24 @@ __global__ __launch_bounds__(kBlock) void count_kernel(uint32_t N, uint32_t ld, 
     // (NaN sorts above infinity, so a non-finite value wins the max)
     uint32_t mx = 0;
     const uint32_t q0 = blockIdx.x * kQueriesPerBlock;
+    // every query's gradient and position loads first, then the index work
+    float g[kQueriesPerThread][F], x[kQueriesPerThread][3];
+    bool live[kQueriesPerThread];
+#pragma unroll
     for (uint32_t k = 0; k < kQueriesPerThread; ++k) {
         const uint32_t q = q0 + k * kBlock + threadIdx.x;
-        if (q >= N) break;
-        float g[F];
-        if (!load_g<F>(d_enc, r.out_row, ld, q, g)) continue;
+        live[k] = q < N;
+        const uint32_t qc = live[k] ? q : N - 1;
+        live[k] = load_g<F>(d_enc, r.out_row, ld, qc, g[k]) && live[k];
 #pragma unroll
-        for (int f = 0; f < F; ++f) mx = max(mx, __float_as_uint(fabsf(g[f])));
-        const float* x = in + (size_t)q * in_stride + r.in_offset;
-        corners(r, x, [&](uint32_t idx, float) { atomicAdd(&cnt[idx / E], 1u); });
+        for (int d = 0; d < 3; ++d) x[k][d] = in[(size_t)qc * in_stride + r.in_offset + d];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kQueriesPerThread; ++k) {
+        if (!live[k]) continue;
+#pragma unroll
+        for (int f = 0; f < F; ++f) mx = max(mx, __float_as_uint(fabsf(g[k][f])));
+        corners(r, x[k], [&](uint32_t idx, float) { atomicAdd(&cnt[idx / E], 1u); });
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_down((int)mx, o, 64));
@@ -292,13 +301,23 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(uint32_t N, uint32_t ld
     }
     __syncthreads();
     const uint32_t q0 = blockIdx.x * kQueriesPerBlock;
+    // every query's gradient and position loads first, then the index work
+    float gq[kQueriesPerThread][F], xq[kQueriesPerThread][3];
+    bool live[kQueriesPerThread];
+#pragma unroll
     for (uint32_t k = 0; k < kQueriesPerThread; ++k) {
         const uint32_t q = q0 + k * kBlock + threadIdx.x;
-        if (q >= N) break;
-        float g[F];
-        if (!load_g<F>(d_enc, r.out_row, ld, q, g)) continue;
-        const float* x = in + (size_t)q * in_stride + r.in_offset;
-        corners(r, x, [&](uint32_t idx, float w) {
+        live[k] = q < N;
+        const uint32_t qc = live[k] ? q : N - 1;
+        live[k] = load_g<F>(d_enc, r.out_row, ld, qc, gq[k]) && live[k];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) xq[k][d] = in[(size_t)qc * in_stride + r.in_offset + d];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kQueriesPerThread; ++k) {
+        if (!live[k]) continue;
+        const float* g = gq[k];
+        corners(r, xq[k], [&](uint32_t idx, float w) {
             const uint32_t slot = atomicAdd(&cursor[idx / E], 1u);
             uint32_t rec[RW];
             rec[0] = idx % E;
