@@ -334,6 +334,13 @@ int nlod_wavefront_run_thresholds(nlod_wavefront* wavefront, void* stream, uint3
 int nlod_wavefront_run_batches(nlod_wavefront* wavefront, void* stream, uint32_t n_batches, const uint32_t* counts,
                                const float* d_vis_in, const float* d_thr_in, float* d_vis_out, float* d_thr_out,
                                int use_graph);
+/* nlod_wavefront_run_batches with one threshold per segment, d_thresholds
+ * float[sum of counts] stored back to back like the other buffers (see
+ * nlod_wavefront_run_thresholds).  The graph is keyed by this buffer too. */
+int nlod_wavefront_run_batches_thresholds(nlod_wavefront* wavefront, void* stream, uint32_t n_batches,
+                                          const uint32_t* counts, const float* d_vis_in, const float* d_thr_in,
+                                          const float* d_thresholds, float* d_vis_out, float* d_thr_out,
+                                          int use_graph);
 
 /* ------------------------------------------ training samples from a mesh */
 

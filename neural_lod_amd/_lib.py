@@ -99,6 +99,7 @@ SIGNATURES: dict[str, list] = {
     "nlod_wavefront_run_shape": [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _i32],
     "nlod_wavefront_run_thresholds": [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _i32],
     "nlod_wavefront_run_batches": [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _i32],
+    "nlod_wavefront_run_batches_thresholds": [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _i32],
     "nlod_labeler_create": [_vp, _vp, _u64, _u32, _u32, C.POINTER(_vp)],
     "nlod_labeler_destroy": [_vp],
     "nlod_labeler_info": [_vp, _u32, C.POINTER(VoxInfo), C.POINTER(_u32), C.POINTER(_u64)],

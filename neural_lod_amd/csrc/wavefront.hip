@@ -322,12 +322,14 @@ struct Wavefront {
     // A sequence of wavefronts with host-known sizes, concatenated in the
     // buffers; optionally captured once per size signature and replayed.
     void run_batches(hipStream_t s, const std::vector<uint32_t>& counts, const float* vin, const float* tin,
-                     float* vout, float* tout, bool use_graph, hipStream_t capture_stream) {
+                     float* vout, float* tout, bool use_graph, hipStream_t capture_stream,
+                     const float* seg_thr = nullptr) {
         auto body = [&](hipStream_t st) {
             size_t off = 0;
             for (uint32_t n : counts) {
                 run(st, n, vin + off * vis.L.n_input_dims, tin + off * thr.L.n_input_dims,
-                    vout + off * vis.L.net.n_output, tout + off * thr.L.net.n_output);
+                    vout + off * vis.L.net.n_output, tout + off * thr.L.net.n_output, -1,
+                    seg_thr ? seg_thr + off : nullptr);
                 off += n;
             }
         };
@@ -336,7 +338,8 @@ struct Wavefront {
             return;
         }
         std::vector<uint32_t> key(counts);
-        for (const void* p : {(const void*)vin, (const void*)tin, (const void*)vout, (const void*)tout}) {
+        for (const void* p : {(const void*)vin, (const void*)tin, (const void*)vout, (const void*)tout,
+                              (const void*)seg_thr}) {
             const uint64_t v = reinterpret_cast<uintptr_t>(p);
             key.push_back((uint32_t)v);
             key.push_back((uint32_t)(v >> 32));
@@ -449,18 +452,20 @@ int nlod_wavefront_max_shape(const nlod_wavefront* wf, int* max_shape_out) {
     });
 }
 
-int nlod_wavefront_run_batches(nlod_wavefront* wf, void* stream, uint32_t n_batches, const uint32_t* counts,
-                               const float* d_vis_in, const float* d_thr_in, float* d_vis_out, float* d_thr_out,
-                               int use_graph) {
+namespace {
+int run_batches_abi(nlod_wavefront* wf, void* stream, uint32_t n_batches, const uint32_t* counts,
+                    const float* d_vis_in, const float* d_thr_in, const float* d_thresholds, float* d_vis_out,
+                    float* d_thr_out, int use_graph, bool need_thresholds) {
     return nlod::guarded([&] {
         NLOD_CHECK(wf && (n_batches == 0 || counts), "null argument");
         std::vector<uint32_t> c(counts, counts + n_batches);
         uint64_t total = 0;
         for (uint32_t v : c) total += v;
         NLOD_CHECK(total == 0 || (d_vis_in && d_thr_in && d_vis_out && d_thr_out), "null argument");
+        NLOD_CHECK(total == 0 || !need_thresholds || d_thresholds, "null argument");
         hipStream_t s = nlod::as_stream(stream);
         if (!use_graph) {
-            wf->w->run_batches(s, c, d_vis_in, d_thr_in, d_vis_out, d_thr_out, false, nullptr);
+            wf->w->run_batches(s, c, d_vis_in, d_thr_in, d_vis_out, d_thr_out, false, nullptr, d_thresholds);
             return;
         }
         if (!wf->capture) {
@@ -470,10 +475,27 @@ int nlod_wavefront_run_batches(nlod_wavefront* wf, void* stream, uint32_t n_batc
         // replay (or capture) on the engine's stream, ordered after and before the caller's work
         NLOD_HIP(hipEventRecord(wf->ev, s));
         NLOD_HIP(hipStreamWaitEvent(wf->capture, wf->ev, 0));
-        wf->w->run_batches(wf->capture, c, d_vis_in, d_thr_in, d_vis_out, d_thr_out, true, wf->capture);
+        wf->w->run_batches(wf->capture, c, d_vis_in, d_thr_in, d_vis_out, d_thr_out, true, wf->capture,
+                           d_thresholds);
         NLOD_HIP(hipEventRecord(wf->ev, wf->capture));
         NLOD_HIP(hipStreamWaitEvent(s, wf->ev, 0));
     });
+}
+}  // namespace
+
+int nlod_wavefront_run_batches(nlod_wavefront* wf, void* stream, uint32_t n_batches, const uint32_t* counts,
+                               const float* d_vis_in, const float* d_thr_in, float* d_vis_out, float* d_thr_out,
+                               int use_graph) {
+    return run_batches_abi(wf, stream, n_batches, counts, d_vis_in, d_thr_in, nullptr, d_vis_out, d_thr_out,
+                           use_graph, false);
+}
+
+int nlod_wavefront_run_batches_thresholds(nlod_wavefront* wf, void* stream, uint32_t n_batches,
+                                          const uint32_t* counts, const float* d_vis_in, const float* d_thr_in,
+                                          const float* d_thresholds, float* d_vis_out, float* d_thr_out,
+                                          int use_graph) {
+    return run_batches_abi(wf, stream, n_batches, counts, d_vis_in, d_thr_in, d_thresholds, d_vis_out, d_thr_out,
+                           use_graph, true);
 }
 
 }  // extern "C"

@@ -405,7 +405,8 @@ class Wavefront:
                       _ptr(vo), _ptr(to), shape)
         return vo, to
 
-    def run_batches(self, counts, vis_in: torch.Tensor, thr_in: torch.Tensor, out=None, graph: bool = False):
+    def run_batches(self, counts, vis_in: torch.Tensor, thr_in: torch.Tensor, out=None, graph: bool = False,
+                    thresholds: torch.Tensor | None = None):
         n = vis_in.shape[0]
         if sum(counts) != n:
             raise ValueError("counts must sum to the number of rows")
@@ -413,6 +414,11 @@ class Wavefront:
         _check_f32(thr_in, "thr_in", (n, self.throughput.n_input_dims))
         vo, to = out if out is not None else self._outputs(n, vis_in.device)
         c = (C.c_uint32 * len(counts))(*counts)
-        _lib.call("nlod_wavefront_run_batches", self._h, _stream(vis_in.device), len(counts), c, _ptr(vis_in),
-                  _ptr(thr_in), _ptr(vo), _ptr(to), int(graph))
+        if thresholds is not None:
+            _check_f32(thresholds, "thresholds", (n,))
+            _lib.call("nlod_wavefront_run_batches_thresholds", self._h, _stream(vis_in.device), len(counts), c,
+                      _ptr(vis_in), _ptr(thr_in), _ptr(thresholds), _ptr(vo), _ptr(to), int(graph))
+        else:
+            _lib.call("nlod_wavefront_run_batches", self._h, _stream(vis_in.device), len(counts), c, _ptr(vis_in),
+                      _ptr(thr_in), _ptr(vo), _ptr(to), int(graph))
         return vo, to

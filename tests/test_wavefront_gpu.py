@@ -138,3 +138,33 @@ def test_per_segment_thresholds(cuda, precision):
         vc, tc = wf.run(vi, ti, shape=shape, thresholds=torch.full((n,), THRESHOLD, device="cuda"))
         vs, ts = wf.run(vi, ti, shape=shape)
         assert torch.equal(vc, vs) and torch.equal(tc, ts), f"constant thresholds differ at shape {shape}"
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batches_per_segment_thresholds(cuda, precision):
+    """Uneven wavefronts with per-segment thresholds: each wavefront equals a
+    single-wavefront run with its slice of the thresholds, eagerly and over
+    several graph replays."""
+    wf, _ = _pair(precision)
+    rng = np.random.default_rng(12)
+    counts = [int(c) for c in np.exp2(rng.uniform(6, 13, 6)).astype(np.int64)]
+    n = sum(counts)
+    print(f"case batches thresholds precision={precision} n={n}", flush=True)
+    vi, ti = _inputs(n, 13)
+    thr = torch.from_numpy(rng.uniform(0.0, 1.0, n).astype(np.float32)).cuda()
+    parts, off = [], 0
+    for c in counts:
+        parts.append(wf.run(vi[off:off + c].contiguous(), ti[off:off + c].contiguous(),
+                            thresholds=thr[off:off + c].contiguous()))
+        off += c
+    ve, te = torch.cat([p[0] for p in parts]), torch.cat([p[1] for p in parts])
+    v0, t0 = wf.run_batches(counts, vi, ti, graph=False, thresholds=thr)
+    assert torch.equal(v0, ve) and torch.equal(t0, te), "eager batches differ from single runs"
+    out = (torch.full_like(ve, -1.0), torch.full_like(te, -1.0))
+    for rep in range(3):
+        out[0].fill_(-1.0)
+        out[1].fill_(-1.0)
+        vg, tg = wf.run_batches(counts, vi, ti, out=out, graph=True, thresholds=thr)
+        torch.cuda.synchronize()
+        assert torch.equal(vg, ve) and torch.equal(tg, te), f"replay {rep} differs"
+        _ = vi.sum().item()
