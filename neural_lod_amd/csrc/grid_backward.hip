@@ -371,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(uint32_t n_bins, const u
     __syncthreads();
     // software-pipelined: the next batch's loads are issued before this
     // batch's LDS adds (lanes past the end load the last record and add 0)
-    uint32_t cur[kInFlight][RW], nxt[kInFlight][RW];
+    uint32_t cur[kInFlight][RW] = {}, nxt[kInFlight][RW] = {};  // nxt stays unread after the last batch
     auto load_batch = [&](uint32_t i0, uint32_t (&dst)[kInFlight][RW]) {
 #pragma unroll
         for (uint32_t k = 0; k < kInFlight; ++k) {
