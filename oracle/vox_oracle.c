@@ -23,6 +23,11 @@
  *   normalize(v)  = v * (1 / sqrt(dot(v,v)))
  *   min/max/clamp = componentwise, clamp(x,lo,hi) = min(max(x,lo),hi)
  *   vec3 -> ivec3 = truncation toward zero
+ * Parity unpinned for this operation order: the reference's CPU path cannot be
+ * compiled here (GLM and trimesh2 are absent) and its repository holds no
+ * voxelized output.  The geometry is pinned (separating-axis brute force on
+ * small meshes, closed-form box shells, analytic solids); the rounding of
+ * voxels whose test sits within float rounding of a boundary is not.
  *
  * Bit layout (util.h:32-45, cpu_voxelizer.cpp:9-17): voxel (x,y,z) has linear
  * index i = x + y*G + z*G*G and lives in bit 31-(i%32) of 32-bit word i/32.
