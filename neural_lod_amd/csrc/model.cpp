@@ -71,9 +71,11 @@ void Model::forward_outputs(hipStream_t s, uint32_t N, const float* in, float* o
     const uint32_t chunk = kRecomputeAbove;
     if (precision == PREC_FP16) {
         ws_feat.ensure(mfma_feature_bytes(L, std::min(N, chunk)));
+        void* pos = pos_scratch(std::min(N, chunk));
         for (uint32_t q0 = 0; q0 < N; q0 += chunk) {
             const uint32_t n = std::min(chunk, N - q0);
-            mfma_encode(s, L, program.p, table_half(use_inference), in + (size_t)q0 * L.n_input_dims, n, ws_feat.p);
+            mfma_encode(s, L, program.p, table_half(use_inference), in + (size_t)q0 * L.n_input_dims, n, ws_feat.p,
+                        pos);
             mfma_mlp_forward(s, L, packed_fwd(use_inference), ws_feat.p, n, out + (size_t)q0 * L.net.n_output);
         }
         return;
@@ -85,6 +87,13 @@ void Model::forward_outputs(hipStream_t s, uint32_t N, const float* in, float* o
         mlp_forward_f32(s, L.net, weights(use_inference), ws_enc.as<float>(), n, out + (size_t)q0 * L.net.n_output,
                         nullptr);
     }
+}
+
+void* Model::pos_scratch(uint32_t N) {
+    const size_t bytes = mfma_pos_scratch_bytes(L, N);
+    if (bytes == 0) return nullptr;
+    ws_pos.ensure(bytes);
+    return ws_pos.p;
 }
 
 void Model::inference(hipStream_t s, uint32_t N, const float* in, float* out) {
@@ -103,7 +112,7 @@ static size_t hidden_bytes(const Layout& L, int precision, uint32_t N) {
 void Model::forward_saved(hipStream_t s, uint32_t N, const float* in, bool use_inference, void* enc, void* hidden,
                           float* out) {
     if (precision == PREC_FP16) {
-        mfma_encode(s, L, program.p, table_half(use_inference), in, N, enc);
+        mfma_encode(s, L, program.p, table_half(use_inference), in, N, enc, pos_scratch(N));
         mfma_mlp_forward_train(s, L, packed_fwd(use_inference), enc, N, hidden, out);
         return;
     }
@@ -232,7 +241,7 @@ uint32_t Model::train_fused(hipStream_t s, uint32_t N, const float* in, const fl
     ws_feat.ensure(mfma_feature_bytes(L, N));
     if (has_grid) ws_denc.ensure(sizeof(float) * (size_t)L.enc_width * N);
     ws_wgrad.ensure(sizeof(float) * (size_t)blocks * L.n_net_params);
-    mfma_encode(s, L, program.p, table_half(false), in, N, ws_feat.p);
+    mfma_encode(s, L, program.p, table_half(false), in, N, ws_feat.p, pos_scratch(N));
     mfma_fused_train(s, L, packed_fwd(false), packed_back(false), ws_feat.p, target, N, loss_type, loss_scale,
                      loss_partial, has_grid ? ws_denc.as<float>() : nullptr, ws_wgrad.as<float>());
     partial_reduce(s, (uint32_t)L.n_net_params, blocks, ws_wgrad.as<float>(), d_params, false);
@@ -396,7 +405,8 @@ int nlod_model_encode_fragments(nlod_model* model, void* stream, uint32_t n, con
         nlod::Model& m = *model->m;
         if (m.precision != nlod::PREC_FP16) nlod::fail(NLOD_ERR_UNSUPPORTED, "FP16 models only");
         m.require_params(true);
-        nlod::mfma_encode(nlod::as_stream(stream), m.L, m.program.p, m.table_half(true), d_in, n, d_features);
+        nlod::mfma_encode(nlod::as_stream(stream), m.L, m.program.p, m.table_half(true), d_in, n, d_features,
+                          m.pos_scratch(n));
     });
 }
 

@@ -43,6 +43,8 @@ struct Model {
     DevBuf packed_inference, packed_bwd_inference, table16_inference;
     DevBuf program;
     DevBuf ws_feat;
+    DevBuf ws_pos;  // [3][N] grid positions of the FP16 encoding pass
+    void* pos_scratch(uint32_t N);  // ws_pos sized for N queries, or null when unused
     // scratch
     DevBuf ws_enc, ws_delta, ws_delta_out, ws_denc, ws_wgrad;
     // recomputed activations of one chunk (batches above kRecomputeAbove)
@@ -119,8 +121,12 @@ void mfma_to_half(hipStream_t stream, const float* src, void* dst, size_t n);
 // bytes of the FP16 feature buffer for N queries ([N/16][K/8][16][8] halves)
 size_t mfma_feature_bytes(const Layout& L, uint32_t N);
 // encoding pass (per-level grid gathers + non-grid rows) -> FP16 feature fragments
+// pos_scratch (nullable): mfma_pos_scratch_bytes(L, N) bytes of device memory
+// for a [3][N] copy of the grid positions; 0 bytes means the layout does not
+// use one.
 void mfma_encode(hipStream_t stream, const Layout& L, const void* d_program, const void* table16, const float* in,
-                 uint32_t N, void* feat);
+                 uint32_t N, void* feat, void* pos_scratch = nullptr);
+size_t mfma_pos_scratch_bytes(const Layout& L, uint32_t N);
 // MLP on MFMA from the feature fragments -> out float[N][n_output]
 void mfma_mlp_forward(hipStream_t stream, const Layout& L, const void* packed, const void* feat, uint32_t N,
                       float* out);
