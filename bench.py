@@ -37,8 +37,8 @@ BYTES_PER_QUERY = ENC_BYTES_PER_QUERY + 48 * 2 + 4
 FLOP_PER_QUERY = 2 * (48 * 64 + 3 * 64 * 64 + 64 * 1)
 # HBM bytes of one encoding pass (its two launches) at 2^20 queries, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes calibrated by a 256 MiB copy
-# (bench.py --profile-encode; 601.5 MB fetched + 135.9 MB written)
-ENC_TRAFFIC_BYTES = 737.4e6
+# (bench.py --profile-encode; 420.5 MB fetched + 150.3 MB written)
+ENC_TRAFFIC_BYTES = 570.8e6
 
 
 def log(msg):
