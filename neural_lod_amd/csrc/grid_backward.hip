@@ -39,7 +39,7 @@ namespace nlod {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kQueriesPerThread = 4;
+constexpr uint32_t kQueriesPerThread = 8;  // count / scatter: loads of 8 queries in flight per thread
 constexpr uint32_t kQueriesPerBlock = kBlock * kQueriesPerThread;
 constexpr uint32_t kTileFloats = 8192;      // values of one tile (kTileFloats / F entries): 64 KiB as int64
 constexpr uint32_t kGroupRecs = 65536;      // records summed by one reduce workgroup
