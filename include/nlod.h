@@ -254,9 +254,12 @@ int nlod_trainer_deserialize(nlod_trainer* trainer, const void* buf, size_t len)
 /* The weight file of the reference (Trainer::serialize without optimizer
  * state, trainer.h:267-278, as write_weights_to_disk writes it,
  * neural_lod_learning_ui.cu:916-924): JSON text
- * {"n_params":N,"params_binary":{"bytes":[...],"subtype":null},"params_type":"float"}
- * holding the inference parameters in tcnn's order.  buf NULL: *needed =
- * size including the terminating NUL.  Synchronizes the trainer's stream. */
+ * {"n_params":N,"params_binary":{"bytes":[...],"subtype":null},"params_type":T}
+ * holding the inference parameters in tcnn's order, in the network precision
+ * as the reference's Trainer<float, T, T> stores them: T "float" (4 bytes per
+ * parameter) for FP32 models, "__half" (2 bytes, rounded to nearest even) for
+ * FP16 models.  buf NULL: *needed = size including the terminating NUL.
+ * Synchronizes the trainer's stream. */
 int nlod_trainer_weights_json(nlod_trainer* trainer, char* buf, size_t buf_len, size_t* needed);
 /* Loads such a file (load_weights_from_disk, neural_lod_learning_ui.cu:
  * 935-947; Trainer::deserialize, trainer.h:280-307): params_type "float" or

@@ -21,6 +21,12 @@ class GridBackward {
     // grads: the full parameter vector.  overwrite: every table entry is stored
     // (entries without contributions get 0); otherwise contributions are added.
     void run(hipStream_t s, const float* in, uint32_t N, const float* d_enc, float* grads, bool overwrite);
+    // attaches the owner's epoch counter to every buffer (see DevBuf)
+    void track(uint64_t* epoch) {
+        for (DevBuf* b : {&d_rows_, &d_bins_, &counts_, &totals_, &bin_start_, &chunk_start_, &meta_, &records_,
+                          &partial_})
+            b->epoch = epoch;
+    }
 
   private:
     uint32_t queries_per_pass() const;

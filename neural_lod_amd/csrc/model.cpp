@@ -16,7 +16,14 @@ namespace nlod {
 Model::Model(uint32_t n_in, uint32_t n_out, const Json& enc, const Json& net, int prec) : precision(prec) {
     if (prec != PREC_FP32 && prec != PREC_FP16) fail(NLOD_ERR_INVALID_ARGUMENT, "precision must be FP32 (0) or FP16 (1)");
     L = build_layout(n_in, n_out, enc, net);
-    if (L.n_params > L.n_net_params) grid_bwd = std::make_unique<GridBackward>(L);
+    for (DevBuf* b : {&packed, &packed_bwd, &table16, &packed_inference, &packed_bwd_inference, &table16_inference,
+                      &program, &ws_feat, &ws_pos, &ws_enc, &ws_delta, &ws_delta_out, &ws_denc, &ws_wgrad,
+                      &ws_hidden, &ws_out})
+        b->epoch = &epoch;
+    if (L.n_params > L.n_net_params) {
+        grid_bwd = std::make_unique<GridBackward>(L);
+        grid_bwd->track(&epoch);
+    }
     if (precision == PREC_FP16) {
         if (!mfma_supported(L)) fail(NLOD_ERR_UNSUPPORTED, "this layout is not supported by the FP16 MFMA kernels");
         program.ensure(mfma_program_bytes());
@@ -26,6 +33,7 @@ Model::Model(uint32_t n_in, uint32_t n_out, const Json& enc, const Json& net, in
 
 void Model::set_params(hipStream_t s, const float* p, const float* p_inf) {
     NLOD_CHECK(p != nullptr, "params pointer is null");
+    ++epoch;  // graphs that read the previous binding (weights or their FP16 copies) are stale
     params = p;
     params_inference = p_inf ? p_inf : p;
     inference_aliased = params_inference == params;

@@ -33,6 +33,10 @@ struct ForwardCtx {
 struct Model {
     Layout L;
     int precision = PREC_FP32;
+    // Incremented when a buffer of the model (or of its grid backward) is freed
+    // or moved, and when the parameter binding changes: a hipGraph that captured
+    // the model's addresses is valid only while the epoch it was stamped with holds.
+    uint64_t epoch = 0;
     const float* params = nullptr;
     const float* params_inference = nullptr;
     bool inference_aliased = true;  // params_inference == params: one set of FP16 copies

@@ -65,16 +65,23 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 inline uint32_t div_round_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 // Grow-only device allocation (never reallocated while a stream may use it
-// within one call; growth happens before launches).
+// within one call; growth happens before launches).  An owner that captures
+// the buffer's address into a hipGraph attaches an epoch counter: every free
+// (including the free of a growth) increments it, so a graph stamped with an
+// older epoch may hold a dangling address and must be captured again.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    uint64_t* epoch = nullptr;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            if (epoch) ++*epoch;
+        }
         p = nullptr;
         bytes = 0;
     }

@@ -112,6 +112,12 @@ struct Trainer {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     std::map<uint32_t, hipGraphExec_t> graphs;
     std::map<uint32_t, int> warm;  // batch sizes that ran once eagerly (workspaces sized)
+    // A captured step holds the addresses of the trainer's step buffers and of
+    // the model's buffers; every graph in `graphs` was captured while both
+    // epochs had the values in graph_stamp.
+    uint64_t own_epoch = 0;
+    std::pair<uint64_t, uint64_t> graph_stamp{0, 0};
+    std::pair<uint64_t, uint64_t> stamp() const { return {model.epoch, own_epoch}; }
 
     Trainer(Model& m, const Json& optimizer, const Json& loss, const Json& scaler, uint32_t seed, bool graphs_on,
             bool fused_on)
@@ -146,6 +152,7 @@ struct Trainer {
         loss_partial.alloc(kLossPartials);
         grad_partial.alloc(kGradPartials);
         bad_partial.alloc(kGradPartials);
+        for (DevBuf* b : {&d_out, &stage_in, &stage_target, &ctx.enc, &ctx.hidden, &ctx.out}) b->epoch = &own_epoch;
         NLOD_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         NLOD_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
         NLOD_HIP(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
@@ -229,16 +236,22 @@ struct Trainer {
             NLOD_HIP(hipMemcpyAsync(stage_target.p, target, tg_bytes, hipMemcpyDeviceToDevice, stream));
             const float* sin = stage_in.as<float>();
             const float* stg = stage_target.as<float>();
+            // a buffer that a graph captured was freed or moved since (a larger
+            // inference or backward call on the model, new parameters, a larger
+            // step): every graph is stale and each size runs eagerly once more
+            if (!graphs.empty() && stamp() != graph_stamp) {
+                drop_graphs();
+                warm.clear();
+            }
             auto it = graphs.find(N);
             if (it != graphs.end()) {
                 NLOD_HIP(hipGraphLaunch(it->second, stream));
             } else if (!warm.count(N)) {
-                // first step at this size: eager, so every workspace gets allocated outside
-                // capture; a workspace that grows invalidates the graphs captured so far
-                drop_graphs();
+                // first step at this size: eager, so every workspace gets allocated outside capture
                 step_body(stream, N, sin, stg);
                 warm[N] = 1;
             } else {
+                const auto before = stamp();
                 hipGraph_t g = nullptr;
                 NLOD_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
                 try {
@@ -253,8 +266,16 @@ struct Trainer {
                 const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(g);
                 NLOD_HIP(e);
-                graphs[N] = exec;
-                NLOD_HIP(hipGraphLaunch(exec, stream));
+                if (stamp() != before) {
+                    // a buffer moved while the body was recorded: the graph may hold a
+                    // freed address, so it is dropped and this step runs eagerly
+                    (void)hipGraphExecDestroy(exec);
+                    step_body(stream, N, sin, stg);
+                } else {
+                    if (graphs.empty()) graph_stamp = before;
+                    graphs[N] = exec;
+                    NLOD_HIP(hipGraphLaunch(exec, stream));
+                }
             }
         }
         NLOD_HIP(hipEventRecord(ev_out, stream));
@@ -343,11 +364,25 @@ struct Trainer {
     // trainer.h:267-278, written as JSON text by write_weights_to_disk,
     // neural_lod_learning_ui.cu:916-924): nlohmann's text form of a binary
     // value is {"bytes": [...], "subtype": null} (gpu_memory_json.h:36-71);
-    // keys sorted as nlohmann's object map sorts them.
+    // keys sorted as nlohmann's object map sorts them.  The payload is the
+    // inference parameters in the network precision, as the reference's
+    // Trainer<float, T, T> stores its T copy: "float" for FP32 models, "__half"
+    // (each value rounded to nearest even) for FP16 models
+    // (neural_lod_learning.h:297 uses T = __half).
     std::string weights_json() {
         NLOD_HIP(hipStreamSynchronize(stream));
+        const bool half = model.precision == PREC_FP16;
         std::vector<uint8_t> bytes(sizeof(float) * n);
         NLOD_HIP(hipMemcpy(bytes.data(), opt.ema ? ema.p() : params.p(), bytes.size(), hipMemcpyDeviceToHost));
+        if (half) {
+            for (size_t k = 0; k < n; ++k) {
+                float f;
+                std::memcpy(&f, bytes.data() + 4 * k, 4);
+                const _Float16 h = (_Float16)f;
+                std::memcpy(bytes.data() + 2 * k, &h, 2);
+            }
+            bytes.resize(2 * n);
+        }
         std::string out;
         out.reserve(bytes.size() * 4 + 128);
         out += "{\"n_params\":" + std::to_string(n) + ",\"params_binary\":{\"bytes\":[";
@@ -361,7 +396,7 @@ struct Trainer {
             num[k++] = (char)('0' + v % 10);
             out.append(num, (size_t)k);
         }
-        out += "],\"subtype\":null},\"params_type\":\"float\"}";
+        out += half ? "],\"subtype\":null},\"params_type\":\"__half\"}" : "],\"subtype\":null},\"params_type\":\"float\"}";
         return out;
     }
 

@@ -1071,25 +1071,21 @@ __global__ __launch_bounds__(kBboxThreads) void bbox_kernel(const float* __restr
     }
 }
 
-int device_count_cached() {
-    static int n = [] {
-        int c = 0;
-        if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
-        return c;
-    }();
-    return n;
-}
-
-// Per-device 6-word accumulator for the bounding-box reduction, allocated once.
-uint32_t* bbox_accumulator() {
+// 6-word accumulator of the bounding-box reduction, one per (device, stream),
+// allocated on first use: calls on different streams (from different host
+// threads) never share one, and calls on one stream are ordered by it.
+uint32_t* bbox_accumulator(hipStream_t s) {
     static std::mutex mu;
-    static std::vector<uint32_t*> per_dev;
+    static std::map<std::pair<int, hipStream_t>, std::unique_ptr<DevBuf>> per;
     int dev = 0;
     NLOD_HIP(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(mu);
-    if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1, nullptr);
-    if (!per_dev[dev]) NLOD_HIP(hipMalloc(&per_dev[dev], 8 * sizeof(uint32_t)));
-    return per_dev[dev];
+    auto& b = per[std::make_pair(dev, s)];
+    if (!b) {
+        b.reset(new DevBuf());
+        b->ensure(8 * sizeof(uint32_t));
+    }
+    return b->as<uint32_t>();
 }
 
 // Tile shape for a G^3 row-major table: T x T rows with T the smallest
@@ -1283,7 +1279,7 @@ void voxelize(hipStream_t stream, const nlod_voxinfo* info, const float* d_tris,
 
 void mesh_bbox(hipStream_t stream, const float* d_tris, uint64_t n_tri, float bmin[3], float bmax[3]) {
     NLOD_CHECK(n_tri > 0 && d_tris != nullptr, "mesh must have at least one triangle");
-    uint32_t* acc = bbox_accumulator();
+    uint32_t* acc = bbox_accumulator(stream);
     hipLaunchKernelGGL(init_bbox_kernel, dim3(1), dim3(64), 0, stream, acc);
     NLOD_LAUNCH_CHECK();
     const bool aligned16 = (reinterpret_cast<uintptr_t>(d_tris) & 15u) == 0;

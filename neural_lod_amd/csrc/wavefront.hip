@@ -31,6 +31,7 @@
 #include <vector>
 
 #include "device_common.h"
+#include "fused_net.h"
 #include "mfma_common.h"
 #include "model.h"
 
@@ -38,34 +39,6 @@ namespace nlod {
 namespace {
 
 constexpr int kBlock = 256;
-
-struct NetArgs {
-    const EncProgram* prog;
-    const half_t* table;   // FP16 copy of the parameter vector (grid tables)
-    uint32_t n_in, n_out, k_padded, n_hidden_layers;
-    int32_t act, out_act;
-    uint32_t packed_halves, lds_offset;  // weights' place in LDS (halves)
-};
-
-template <int W, int F, int MAXD>
-__device__ __forceinline__ floatx4 run_net(const NetArgs& a, const half_t* __restrict__ sw, const float* __restrict__ x) {
-    constexpr int MT = W / 16;
-    const int lane = threadIdx.x & 63, g = lane >> 4;
-    const half_t* w = sw + a.lds_offset;
-    const uint32_t kc0 = a.k_padded / 32;
-    floatx4 acc[MT];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) acc[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (uint32_t c = 0; c < kc0; ++c) {
-        const half8 b = encode_window<F, MAXD, half_t>(*a.prog, a.table, x, 32 * c + 8 * g);
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            const half8 wf = *reinterpret_cast<const half8*>(w + ((size_t)(mt * kc0 + c) * 64 + lane) * 8);
-            acc[mt] = mfma16(wf, b, acc[mt]);
-        }
-    }
-    return mlp_tail16<W>(w + (size_t)MT * kc0 * 512, acc, a.n_hidden_layers, a.act);
-}
 
 template <int W, int F, int MAXD>
 __global__ __launch_bounds__(kBlock) void wavefront_kernel(uint32_t N, NetArgs vis, NetArgs thr, float threshold,
@@ -129,38 +102,6 @@ __global__ void gate_kernel(uint32_t N, uint32_t n_vis, uint32_t n_thr, float th
         for (uint32_t i = 0; i < n_thr; ++i) thr_out[(size_t)q * n_thr + i] = 0.0f;
 }
 
-template <typename Fn>
-void dispatch_wf(uint32_t W, uint32_t F, uint32_t D, Fn&& f) {
-    auto with_d = [&](auto Wc, auto Fc) {
-        if (D <= 4) f(Wc, Fc, std::integral_constant<int, 4>{});
-        else f(Wc, Fc, std::integral_constant<int, 8>{});
-    };
-    auto with_f = [&](auto Wc) {
-        switch (F) {
-            case 1: with_d(Wc, std::integral_constant<int, 1>{}); break;
-            case 2: with_d(Wc, std::integral_constant<int, 2>{}); break;
-            case 4: with_d(Wc, std::integral_constant<int, 4>{}); break;
-            case 8: with_d(Wc, std::integral_constant<int, 8>{}); break;
-            default: fail(NLOD_ERR_UNSUPPORTED, "unsupported n_features_per_level");
-        }
-    };
-    switch (W) {
-        case 32: with_f(std::integral_constant<int, 32>{}); break;
-        case 64: with_f(std::integral_constant<int, 64>{}); break;
-        default: fail(NLOD_ERR_UNSUPPORTED, "fused wavefront kernel supports widths 32 and 64");
-    }
-}
-
-void layout_features(const Layout& L, uint32_t& F, uint32_t& maxdeg) {
-    F = 0;
-    maxdeg = 0;
-    for (const Segment& s : L.segs) {
-        if (s.type == SEG_GRID) F = s.F;
-        if (s.type == SEG_SH) maxdeg = std::max(maxdeg, s.degree);
-    }
-    if (F == 0) F = 2;
-}
-
 // wavefront-size buckets of the autotuner: [2^k, 2^(k+2)) for k = 8, 10, ..., 20
 constexpr int kBuckets = 8;
 int bucket_of(uint32_t n) {
@@ -183,7 +124,24 @@ struct Wavefront {
     bool tuned = false;
     int max_blocks_per_cu = 1;
     DevBuf tune_in_v, tune_in_t, tune_out_v, tune_out_t;
-    std::map<std::vector<uint32_t>, hipGraphExec_t> graphs;
+    // Captured sequences, keyed by counts and buffers, with the tick of their
+    // last use (least recently used goes first beyond kMaxGraphs).  Every graph
+    // was captured while both models' epochs had the values in graph_stamp: a
+    // replay after one of them moved a buffer or rebound its weights would read
+    // freed or stale memory, so a changed epoch drops them all.
+    static constexpr size_t kMaxGraphs = 16;
+    struct CachedGraph {
+        hipGraphExec_t exec;
+        uint64_t last_use;
+    };
+    std::map<std::vector<uint32_t>, CachedGraph> graphs;
+    uint64_t use_tick = 0;
+    std::pair<uint64_t, uint64_t> graph_stamp{0, 0};
+    std::pair<uint64_t, uint64_t> stamp() const { return {vis.epoch, thr.epoch}; }
+    void drop_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second.exec);
+        graphs.clear();
+    }
 
     Wavefront(Model& v, Model& t, float thr_) : vis(v), thr(t), threshold(thr_) {
         if (v.precision != t.precision) fail(NLOD_ERR_INVALID_ARGUMENT, "both networks must use the same precision");
@@ -209,24 +167,9 @@ struct Wavefront {
         }
     }
 
-    ~Wavefront() {
-        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
-    }
+    ~Wavefront() { drop_graphs(); }
 
-    NetArgs args_of(const Model& m, uint32_t lds_offset) const {
-        NetArgs a{};
-        a.prog = static_cast<const EncProgram*>(m.program.p);
-        a.table = static_cast<const half_t*>(m.table_half(true));
-        a.n_in = m.L.n_input_dims;
-        a.n_out = m.L.net.n_output;
-        a.k_padded = k_padded_of(m.L.net);
-        a.n_hidden_layers = m.L.net.n_hidden_layers;
-        a.act = m.L.net.activation;
-        a.out_act = m.L.net.output_activation;
-        a.packed_halves = (uint32_t)(mfma_packed_bytes(m.L.net) / sizeof(half_t));
-        a.lds_offset = lds_offset;
-        return a;
-    }
+    NetArgs args_of(const Model& m, uint32_t lds_offset) const { return net_args_of(m, lds_offset); }
 
     void run_fused(hipStream_t s, uint32_t N, const float* vin, const float* tin, float* vout, float* tout,
                    int blocks_per_cu, const float* seg_thr = nullptr) {
@@ -315,8 +258,7 @@ struct Wavefront {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         tuned = true;
-        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
-        graphs.clear();
+        drop_graphs();
     }
 
     // A sequence of wavefronts with host-known sizes, concatenated in the
@@ -344,9 +286,11 @@ struct Wavefront {
             key.push_back((uint32_t)v);
             key.push_back((uint32_t)(v >> 32));
         }
+        if (!graphs.empty() && stamp() != graph_stamp) drop_graphs();
         auto it = graphs.find(key);
         if (it == graphs.end()) {
             body(s);  // eager first: workspaces of the split path get allocated outside capture
+            const auto before = stamp();
             hipGraph_t g = nullptr;
             NLOD_HIP(hipStreamBeginCapture(capture_stream, hipStreamCaptureModeRelaxed));
             try {
@@ -361,10 +305,23 @@ struct Wavefront {
             const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             NLOD_HIP(e);
-            it = graphs.emplace(key, exec).first;
+            if (stamp() != before) {  // a buffer moved during capture: not kept
+                (void)hipGraphExecDestroy(exec);
+                return;
+            }
+            if (graphs.empty()) graph_stamp = before;
+            if (graphs.size() >= kMaxGraphs) {
+                auto lru = graphs.begin();
+                for (auto j = graphs.begin(); j != graphs.end(); ++j)
+                    if (j->second.last_use < lru->second.last_use) lru = j;
+                (void)hipGraphExecDestroy(lru->second.exec);
+                graphs.erase(lru);
+            }
+            graphs.emplace(key, CachedGraph{exec, ++use_tick});
             return;  // this call's results came from the eager run
         }
-        NLOD_HIP(hipGraphLaunch(it->second, s));
+        it->second.last_use = ++use_tick;
+        NLOD_HIP(hipGraphLaunch(it->second.exec, s));
     }
 };
 

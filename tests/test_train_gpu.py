@@ -195,6 +195,34 @@ def test_trainer_graph_replay_matches_eager(cuda, precision):
     assert close.float().mean().item() >= 0.999
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_trainer_graph_after_model_buffers_grow(cuda, precision):
+    """A captured step holds the addresses of the model's scratch buffers.
+    Inference on the same model at larger batches between steps (as the
+    reference does between training steps, neural_lod_learning.cu:379,781)
+    frees and moves those buffers; the steps after it must not replay the old
+    graph.  The graph trainer must track an eager trainer given the same calls."""
+    cfg = SMALL_VIS
+    eager = _trainer(cfg, precision, seed=11)
+    graph = _trainer(cfg, precision, seed=11, graphs=True)
+    big = torch.from_numpy(synthetic.visibility_queries(1 << 16, 99)).cuda()
+    for k in range(10):
+        x, t = _batches(512, 1, 200 + k)[0]
+        xt, tt = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
+        eager.training_step(xt, tt)
+        graph.training_step(xt, tt)
+        a, b = eager.status(), graph.status()
+        assert b["loss"] == pytest.approx(a["loss"], rel=1e-5, abs=1e-9), k
+        if k in (3, 6):  # the graph was captured at step 1 and replayed since
+            n = 1 << (13 + k // 3)
+            ye = eager.model.inference(big[:n])
+            yg = graph.model.inference(big[:n])
+            assert torch.equal(ye, yg) or (ye - yg).abs().max().item() <= 1e-5
+    pa, pb = eager.params(), graph.params()
+    close = (pa - pb).abs() <= 1e-5 + 1e-4 * pa.abs()
+    assert close.float().mean().item() >= 0.999
+
+
 @pytest.mark.parametrize("name", ["visibility", "throughput", "small_vis"])
 def test_fused_training_pass_matches_unfused(cuda, name):
     """FP16: the fused forward + loss + backward + dW kernel gives the loss and
@@ -353,3 +381,25 @@ def test_weights_json_roundtrip(cuda):
     d16["params_binary"]["bytes"] = d16["params_binary"]["bytes"][:-2]
     with pytest.raises(NlodError, match="parameters"):
         tr2.load_weights_json(json.dumps(d16))
+
+
+def test_weights_json_fp16_model_writes_half(cuda):
+    """FP16 models write the reference's Trainer<float, __half, __half> file:
+    params_type "__half", 2 bytes per parameter, the inference parameters
+    rounded to nearest even; it loads back into an FP16 and an FP32 trainer."""
+    import json
+    cfg = SMALL_VIS
+    tr = _trainer(cfg, 1, seed=4)
+    for x, t in _batches(512, 3, 50):
+        tr.training_step(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda())
+    text = tr.weights_json()
+    assert text.endswith('"params_type":"__half"}')
+    d = json.loads(text)
+    h = np.array(d["params_binary"]["bytes"], np.uint8).view(np.float16)
+    assert d["n_params"] == h.size == tr.model.n_params
+    ref = tr.inference_params().cpu().numpy().astype(np.float16)
+    assert np.array_equal(h.view(np.uint16), ref.view(np.uint16))
+    for prec in (1, 0):
+        tr2 = _trainer(cfg, prec, seed=98)
+        tr2.load_weights_json(text)
+        assert np.array_equal(tr2.params().cpu().numpy(), h.astype(np.float32))

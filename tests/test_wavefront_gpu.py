@@ -100,6 +100,36 @@ def test_batches_graph_replay(cuda, precision):
         _ = vi.sum().item()  # eager work between replays
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batches_graph_after_rebind_and_growth(cuda, precision):
+    """A captured sequence holds the models' weight and scratch addresses.
+    After new parameters are bound to one model, and after a larger
+    inference call moves the other's scratch buffers, a graph call must give
+    what an eager run with the new parameters gives."""
+    wf, nets = _pair(precision)
+    rng = np.random.default_rng(12)
+    counts = [int(c) for c in np.exp2(rng.uniform(6, 13, 5)).astype(np.int64)]
+    vi, ti = _inputs(sum(counts), 13)
+    v0, t0 = wf.run_batches(counts, vi, ti, graph=False)
+    out = (torch.empty_like(v0), torch.empty_like(t0))
+    for _ in range(2):  # capture, then replay
+        wf.run_batches(counts, vi, ti, out=out, graph=True)
+    torch.cuda.synchronize()
+    (mv, _, _, _), (mt, Lt, pt_, _) = nets
+    p2 = torch.from_numpy(M.xavier_uniform(Lt, 77, grid_range=1.0)).cuda()
+    mt.set_params(p2)
+    big = torch.from_numpy(synthetic.visibility_queries(1 << 17, 14)).cuda()
+    _ = mv.inference(big)
+    ve, te = wf.run_batches(counts, vi, ti, graph=False)
+    assert not torch.equal(te, t0), "the new throughput weights must change the outputs"
+    for rep in range(3):
+        out[0].fill_(-1.0)
+        out[1].fill_(-1.0)
+        vg, tg = wf.run_batches(counts, vi, ti, out=out, graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(vg, ve) and torch.equal(tg, te), f"call {rep} after the rebind differs"
+
+
 def test_autotuner(cuda):
     wf, _ = _pair(1)
     vi, ti = _inputs(1 << 16, 10)
