@@ -101,6 +101,20 @@ int nlod_voxelize(void* stream, const nlod_voxinfo* info, const float* d_triangl
 int nlod_voxelize_mesh(void* stream, const float* d_triangles, uint64_t n_triangles, uint32_t res,
                        uint32_t flags, uint32_t* d_table, nlod_voxinfo* info_out);
 
+/* nlod_voxelize that allocates nothing: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * flags and bits as nlod_voxelize; the LDS-tiled path takes its scratch from a
+ * fixed device array of 32 MiB per GPU (held until the process ends) instead
+ * of a library allocation.  A job whose tiled path needs more scratch than
+ * that fails with NLOD_ERR_INVALID_ARGUMENT (about 550k triangles at 256^3,
+ * 200k at 1024^3); NLOD_VOX_DIRECT needs no scratch at any size.  Every call
+ * on one GPU shares its array: calls that overlap on one GPU (on different
+ * streams or threads) race and must be serialized by the caller.
+ * Asynchronous. */
+int nlod_voxelize_noalloc(void* stream, const nlod_voxinfo* info, const float* d_triangles, uint32_t* d_table,
+                          uint32_t flags);
+
 /* ----------------------------------------- network with input encoding */
 
 typedef struct nlod_model nlod_model;

@@ -63,6 +63,7 @@ SIGNATURES: dict[str, list] = {
     "nlod_mesh_bbox": [_vp, _vp, _u64, _fp, _fp],
     "nlod_voxelize": [_vp, C.POINTER(VoxInfo), _vp, _vp, _u32],
     "nlod_voxelize_mesh": [_vp, _vp, _u64, _u32, _u32, _vp, C.POINTER(VoxInfo)],
+    "nlod_voxelize_noalloc": [_vp, C.POINTER(VoxInfo), _vp, _vp, _u32],
     "nlod_model_create": [_u32, _u32, C.c_char_p, C.c_char_p, _i32, C.POINTER(_vp)],
     "nlod_model_destroy": [_vp],
     "nlod_model_get_info": [_vp, _vp],

@@ -41,8 +41,10 @@ size_t mlp_weight_grad_workspace_floats(const NetShape& net, uint32_t N);
 // ---- voxelize.hip -------------------------------------------------------------
 void voxinfo_from_bbox(const float bmin[3], const float bmax[3], uint32_t G, uint32_t flags, uint64_t n_tri,
                        struct nlod_voxinfo* info);
+// no_alloc: the tiled path takes its scratch from the library's fixed device
+// array instead of the per-stream buffer (fails when the job needs more)
 void voxelize(hipStream_t stream, const struct nlod_voxinfo* info, const float* d_tris, uint32_t* d_table,
-              uint32_t flags);
+              uint32_t flags, bool no_alloc = false);
 void mesh_bbox(hipStream_t stream, const float* d_tris, uint64_t n_tri, float bmin[3], float bmax[3]);
 
 }  // namespace nlod

@@ -1148,19 +1148,28 @@ void launch_direct(hipStream_t stream, const VoxParams& p, const float* d_tris, 
     NLOD_LAUNCH_CHECK();
 }
 
-template <bool SOLID>
+// Bytes of tiled-path scratch for a job of this geometry: routes (u64 per
+// triangle) | counts | totals | start | chunk_start (n_tiles + 1) | idx.
+size_t tile_scratch_bytes(const VoxParams& p, const TileGeo& geo) {
+    const uint32_t n_blocks = (uint32_t)((p.n_tri + geo.tris_per_block - 1) / geo.tris_per_block);
+    const size_t n_counts = (size_t)geo.n_tiles * n_blocks;
+    const size_t n_idx = (size_t)p.n_tri * kMaxTiles;
+    return (2 * (size_t)p.n_tri + n_counts + 3 * (size_t)geo.n_tiles + 1 + n_idx) * sizeof(uint32_t);
+}
+
+// The tiled path's scratch comes from `workspace(bytes)`: the library's
+// per-stream buffer, or the caller-free static array of nlod_voxelize_noalloc.
+// Every word of it that a kernel reads was written earlier in the same job.
+template <bool SOLID, typename Workspace>
 void launch_voxelize(hipStream_t stream, const VoxParams& p, const TileGeo& geo, bool tiled, const float* d_tris,
-                     uint32_t* d_table) {
+                     uint32_t* d_table, Workspace&& workspace) {
     if (!tiled) {
         launch_direct<SOLID>(stream, p, d_tris, nullptr, d_table);
         return;
     }
     const uint32_t n_blocks = (uint32_t)((p.n_tri + geo.tris_per_block - 1) / geo.tris_per_block);
     const size_t n_counts = (size_t)geo.n_tiles * n_blocks;
-    const size_t n_idx = (size_t)p.n_tri * kMaxTiles;
-    // routes (u64 per triangle) | counts | totals | start | chunk_start (n_tiles + 1) | idx
-    const size_t words = 2 * (size_t)p.n_tri + n_counts + 3 * (size_t)geo.n_tiles + 1 + n_idx;
-    uint8_t* ws = tile_workspace(stream, words * sizeof(uint32_t));
+    uint8_t* ws = workspace(tile_scratch_bytes(p, geo));
     uint64_t* routes = reinterpret_cast<uint64_t*>(ws);
     uint32_t* counts = reinterpret_cast<uint32_t*>(routes + p.n_tri);
     uint32_t* totals = counts + n_counts;
@@ -1247,19 +1256,38 @@ void voxinfo_from_bbox(const float bmin[3], const float bmax[3], uint32_t G, uin
     info->n_triangles = n_tri;
 }
 
-void voxelize(hipStream_t stream, const nlod_voxinfo* info, const float* d_tris, uint32_t* d_table, uint32_t flags) {
+// File-scope scratch of nlod_voxelize_noalloc: one copy per device, held until
+// the process ends.  Host code never passes `scratch` itself to a kernel (in
+// host code the name is a stand-in, not a device address); it uses the address
+// hipGetSymbolAddress returns on each device.
+constexpr size_t kNoAllocScratchBytes = 32u << 20;
+__device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+uint8_t* noalloc_scratch() {
+    static std::mutex mu;
+    static std::vector<uint8_t*> per_dev;
+    int dev = 0;
+    NLOD_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1, nullptr);
+    if (!per_dev[dev]) {
+        void* ptr = nullptr;
+        NLOD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        NLOD_CHECK(ptr != nullptr, "hipGetSymbolAddress returned no address for the voxelizer scratch");
+        per_dev[dev] = static_cast<uint8_t*>(ptr);
+    }
+    return per_dev[dev];
+}
+
+void voxelize(hipStream_t stream, const nlod_voxinfo* info, const float* d_tris, uint32_t* d_table, uint32_t flags,
+              bool no_alloc) {
     validate_info(info, flags);
     NLOD_CHECK(d_table != nullptr, "voxel table pointer is null");
     NLOD_CHECK(info->n_triangles == 0 || d_tris != nullptr, "triangle pointer is null");
     const uint64_t G = info->gridsize[0];
     const uint64_t words = (G * G * G + 31) / 32;
-    if (!(flags & NLOD_VOX_ACCUMULATE)) {
-        const uint32_t blocks = (uint32_t)std::min<uint64_t>(div_round_up(words / 4 + 1, 256), 8192);
-        hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, stream, d_table, words,
-                           (int)(((uintptr_t)d_table & 15u) == 0));
-        NLOD_LAUNCH_CHECK();
-    }
-    if (info->n_triangles == 0) return;
     VoxParams p{};
     for (int i = 0; i < 3; ++i) { p.bmin[i] = info->bbox_min[i]; p.unit[i] = info->unit[i]; }
     p.G = (uint32_t)G;
@@ -1270,11 +1298,28 @@ void voxelize(hipStream_t stream, const nlod_voxinfo* info, const float* d_tris,
     // the bound of the direct kernel); the surface test is compute-bound
     // either way and the direct kernel has no binning passes
     const bool want = (flags & NLOD_VOX_TILED) || ((flags & NLOD_VOX_SOLID) && !(flags & NLOD_VOX_DIRECT));
-    const bool tiled = !p.morton && want &&
+    const bool tiled = p.n_tri > 0 && !p.morton && want &&
                        tile_geometry(p.G, p.n_tri, (flags & NLOD_VOX_TILED) != 0, (flags & NLOD_VOX_ACCUMULATE) != 0,
                                      geo);
-    if (flags & NLOD_VOX_SOLID) launch_voxelize<true>(stream, p, geo, tiled, d_tris, d_table);
-    else launch_voxelize<false>(stream, p, geo, tiled, d_tris, d_table);
+    uint8_t* fixed = nullptr;
+    if (no_alloc) {  // checked before any launch: a rejected call leaves the table untouched
+        if (tiled && tile_scratch_bytes(p, geo) > kNoAllocScratchBytes)
+            fail(NLOD_ERR_INVALID_ARGUMENT,
+                 "nlod_voxelize_noalloc: this job needs " + std::to_string(tile_scratch_bytes(p, geo)) +
+                     " bytes of tiled-path scratch, more than the " + std::to_string(kNoAllocScratchBytes) +
+                     " bytes it holds (pass NLOD_VOX_DIRECT, which needs none, or use nlod_voxelize)");
+        fixed = noalloc_scratch();
+    }
+    if (!(flags & NLOD_VOX_ACCUMULATE)) {
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>(div_round_up(words / 4 + 1, 256), 8192);
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, stream, d_table, words,
+                           (int)(((uintptr_t)d_table & 15u) == 0));
+        NLOD_LAUNCH_CHECK();
+    }
+    if (p.n_tri == 0) return;
+    auto workspace = [&](size_t bytes) { return no_alloc ? fixed : tile_workspace(stream, bytes); };
+    if (flags & NLOD_VOX_SOLID) launch_voxelize<true>(stream, p, geo, tiled, d_tris, d_table, workspace);
+    else launch_voxelize<false>(stream, p, geo, tiled, d_tris, d_table, workspace);
 }
 
 void mesh_bbox(hipStream_t stream, const float* d_tris, uint64_t n_tri, float bmin[3], float bmax[3]) {
@@ -1330,7 +1375,12 @@ int nlod_mesh_bbox(void* stream, const float* d_triangles, uint64_t n_triangles,
 
 int nlod_voxelize(void* stream, const nlod_voxinfo* info, const float* d_triangles, uint32_t* d_table,
                   uint32_t flags) {
-    return nlod::guarded([&] { nlod::voxelize(nlod::as_stream(stream), info, d_triangles, d_table, flags); });
+    return nlod::guarded([&] { nlod::voxelize(nlod::as_stream(stream), info, d_triangles, d_table, flags, false); });
+}
+
+int nlod_voxelize_noalloc(void* stream, const nlod_voxinfo* info, const float* d_triangles, uint32_t* d_table,
+                          uint32_t flags) {
+    return nlod::guarded([&] { nlod::voxelize(nlod::as_stream(stream), info, d_triangles, d_table, flags, true); });
 }
 
 int nlod_voxelize_mesh(void* stream, const float* d_triangles, uint64_t n_triangles, uint32_t res, uint32_t flags,
@@ -1340,7 +1390,7 @@ int nlod_voxelize_mesh(void* stream, const float* d_triangles, uint64_t n_triang
         nlod::mesh_bbox(nlod::as_stream(stream), d_triangles, n_triangles, bmin, bmax);
         nlod_voxinfo info;
         nlod::voxinfo_from_bbox(bmin, bmax, res, flags, n_triangles, &info);
-        nlod::voxelize(nlod::as_stream(stream), &info, d_triangles, d_table, flags);
+        nlod::voxelize(nlod::as_stream(stream), &info, d_triangles, d_table, flags, false);
         if (info_out) *info_out = info;
     });
 }
