@@ -202,6 +202,11 @@ int nlod_model_encode_backward(nlod_model* model, void* stream, uint32_t n, cons
  * reuse one encoding across several networks; no counterpart in the reference. */
 int nlod_model_feature_bytes(const nlod_model* model, uint32_t n, uint64_t* bytes_out);
 int nlod_model_encode_fragments(nlod_model* model, void* stream, uint32_t n, const float* d_in, void* d_features);
+/* FP16 models: the fused MLP alone on features that nlod_model_encode_fragments
+ * wrote, -> d_out float[n][output_width] (inference parameters).  The second
+ * half of nlod_model_inference, for profiling and for callers that evaluate
+ * one encoding once; no counterpart in the reference. */
+int nlod_model_mlp_fragments(nlod_model* model, void* stream, uint32_t n, const void* d_features, float* d_out);
 
 /* ---------------------------------------------------------------- trainer */
 
@@ -455,6 +460,77 @@ int nlod_lod_grid_march_thresholds(const nlod_lod_grid* grid, void* stream, uint
                                    const float* d_origins, const float* d_dirs, const float* d_t_min,
                                    const uint32_t* d_prev_voxel, uint32_t* d_voxel_out, float* d_t_near_out,
                                    float* d_t_far_out, float* d_threshold_out);
+
+/* ------------------------------------------------ neural LoD path tracer */
+
+/* Pinhole camera of the reference's ViewParams (rptr/vulkan/gpu_params.glsl:
+ * 70-74): pixel (x, y) of a width x height framebuffer looks along
+ * normalize((x + u) / width * du + (y + v) / height * dv + dir_top_left) from
+ * position, (u, v) a per-pixel jitter (camera_raygen,
+ * rptr/cuda/wavefront_neural_pt.cu:9-49).  World space. */
+typedef struct nlod_camera {
+    float position[3];
+    float dir_top_left[3];
+    float du[3];
+    float dv[3];
+    uint32_t width, height;
+} nlod_camera;
+
+enum nlod_tracer_schedule {
+    NLOD_TRACER_AUTO = 0,       /* persistent where it applies, else the reference's */
+    NLOD_TRACER_REFERENCE = 1,  /* the reference's wavefront rounds, one host count readback per round */
+    NLOD_TRACER_PERSISTENT = 2  /* FP16: one launch per frame, device ray queue, no host readback */
+};
+
+/* WavefrontNeuralThroughputVisibilityLod parameters
+ * (rptr/cuda/neural-lod/wavefront_neural_throughput_visibility_lod.h:19-31);
+ * nlod_tracer_default_params fills the reference's defaults. */
+typedef struct nlod_tracer_params {
+    uint32_t lod;                        /* pyramid level marched (current_lod) */
+    uint32_t max_depth;                  /* bounces (3) */
+    float background_color[3];           /* paths escaping at bounce 0 (1, 1, 1) */
+    float envmap_color[3];               /* constant environment, later bounces (1, 1, 1) */
+    float inference_min_extent_dilation; /* 1e-4 */
+    float voxel_extent_dilation_outward; /* dilation the visibility network was trained with (1e-4) */
+    uint32_t apply_rr, rr_start_bounce;  /* Russian roulette on the throughput (off, 1) */
+    uint32_t apply_visibility_rr;        /* roulette on unoccluded visibility rounds >= 5 (off) */
+    uint32_t stochastic_threshold;       /* occluded when visibility >= a uniform draw (off) */
+    uint32_t use_hg;                     /* Henyey-Greenstein directions for the throughput query (off) */
+    float hg_g;                          /* 0.5 */
+    uint32_t throughput_log_learning;    /* outputs are log(1 + throughput) (off) */
+    uint32_t schedule;                   /* enum nlod_tracer_schedule */
+} nlod_tracer_params;
+
+typedef struct nlod_tracer nlod_tracer;
+
+int nlod_tracer_default_params(nlod_tracer_params* params_out);
+
+/* Path tracer over one level of a LoD pyramid with a visibility network of 6
+ * inputs (the segment's entry and exit points in the voxel, normalized to the
+ * box) and a throughput network of 9 inputs (voxel centre, (wo + 1) / 2,
+ * (wi + 1) / 2) or 10 (the level appended) and 3 outputs.  The grid and the
+ * models must outlive the tracer; the networks' inference parameters are used.
+ * Replaces the WavefrontNeuralThroughputVisibilityLod render extension
+ * (wavefront_neural_throughput_visibility_lod.cu:420-462, 727-794). */
+int nlod_tracer_create(const nlod_lod_grid* grid, nlod_model* visibility, nlod_model* throughput,
+                       nlod_tracer** tracer_out);
+int nlod_tracer_destroy(nlod_tracer* tracer);
+
+/* One sample per pixel: voxel_tracing(sample_idx, ...)
+ * (wavefront_neural_throughput_visibility_lod.cu:464-720) with random
+ * numbers from get_lcg_rng(sample_idx, sample_offset, pixel).  Per-voxel
+ * thresholds come from nlod_lod_grid_set_thresholds (0.5 where unset).
+ * d_illumination float[width * height][3] gets each path's contribution
+ * (what accumulate_neural_bounce writes), d_inferences (nullable)
+ * uint32[width * height] its visibility-network queries (needed_inferences),
+ * d_bounces (nullable) the bounce at which it ended.  The reference schedule
+ * synchronizes the stream once per round; the persistent one is asynchronous,
+ * allocates nothing after its first call and can be captured in a hipGraph. */
+int nlod_tracer_render(nlod_tracer* tracer, void* stream, const nlod_camera* camera,
+                       const nlod_tracer_params* params, uint32_t sample_idx, uint32_t sample_offset,
+                       float* d_illumination, uint32_t* d_inferences, uint32_t* d_bounces);
+/* The schedule nlod_tracer_render would use for these parameters. */
+int nlod_tracer_schedule(const nlod_tracer* tracer, const nlod_tracer_params* params, int* schedule_out);
 
 /* ------------------------------------------ per-voxel visibility thresholds */
 

@@ -78,6 +78,7 @@ SIGNATURES: dict[str, list] = {
     "nlod_model_encode_backward": [_vp, _vp, _u32, _vp, _vp, _vp, _i32],
     "nlod_model_feature_bytes": [_vp, _u32, C.POINTER(_u64)],
     "nlod_model_encode_fragments": [_vp, _vp, _u32, _vp, _vp],
+    "nlod_model_mlp_fragments": [_vp, _vp, _u32, _vp, _vp],
     "nlod_trainer_create": [_vp, C.c_char_p, C.c_char_p, C.c_char_p, _u32, _u32, C.POINTER(_vp)],
     "nlod_trainer_destroy": [_vp],
     "nlod_trainer_step": [_vp, _vp, _u32, _vp, _vp],
@@ -114,6 +115,11 @@ SIGNATURES: dict[str, list] = {
     "nlod_lod_grid_active_voxels": [_vp, _vp, _u32, C.POINTER(_u64)],
     "nlod_lod_grid_set_thresholds": [_vp, _vp, _u32, _u64, _vp, C.c_float, C.c_float],
     "nlod_optimal_thresholds": [_vp, _u32, _u32, _vp, _vp, _u32, _i32, C.c_float, _vp, _vp],
+    "nlod_tracer_default_params": [_vp],
+    "nlod_tracer_create": [_vp, _vp, _vp, C.POINTER(_vp)],
+    "nlod_tracer_destroy": [_vp],
+    "nlod_tracer_render": [_vp, _vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp],
+    "nlod_tracer_schedule": [_vp, _vp, C.POINTER(_i32)],
 }
 
 

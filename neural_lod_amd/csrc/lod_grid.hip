@@ -390,11 +390,34 @@ void lod_grid_march(const LodGrid& g, hipStream_t s, uint32_t lod, uint32_t n, c
     NLOD_LAUNCH_CHECK();
 }
 
+// What the path tracer reads of a level: bits, dense thresholds (null when
+// unset), resolution, box, the level-0 voxel edge and resolution.
+void lod_grid_tracer_view(const LodGrid& g, uint32_t lod, const uint32_t** bits, const float** thr, uint32_t* res,
+                          float bmin[3], float bmax[3], float* voxel_size, uint32_t* grid_res) {
+    NLOD_CHECK(lod < g.n_lods, "lod out of range");
+    *bits = g.bits[lod].as<uint32_t>();
+    *thr = g.thr[lod].as<float>();
+    *res = g.res[lod];
+    for (int k = 0; k < 3; ++k) {
+        bmin[k] = g.bmin[k];
+        bmax[k] = g.bmax[k];
+    }
+    *voxel_size = g.unit[0][0];
+    *grid_res = g.res[0];
+}
+
 }  // namespace nlod
 
 struct nlod_lod_grid {
     std::unique_ptr<nlod::LodGrid> impl;
 };
+
+namespace nlod {
+const LodGrid& lod_grid_of(const nlod_lod_grid* g) {
+    NLOD_CHECK(g && g->impl, "null grid");
+    return *g->impl;
+}
+}  // namespace nlod
 
 extern "C" {
 

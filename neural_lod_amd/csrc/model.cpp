@@ -418,6 +418,16 @@ int nlod_model_encode_fragments(nlod_model* model, void* stream, uint32_t n, con
     });
 }
 
+int nlod_model_mlp_fragments(nlod_model* model, void* stream, uint32_t n, const void* d_features, float* d_out) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(model && (n == 0 || (d_features && d_out)), "null argument");
+        nlod::Model& m = *model->m;
+        if (m.precision != nlod::PREC_FP16) nlod::fail(NLOD_ERR_UNSUPPORTED, "FP16 models only");
+        m.require_params(true);
+        if (n) nlod::mfma_mlp_forward(nlod::as_stream(stream), m.L, m.packed_fwd(true), d_features, n, d_out);
+    });
+}
+
 int nlod_model_inference(nlod_model* model, void* stream, uint32_t n, const float* d_in, float* d_out) {
     return nlod::guarded([&] {
         NLOD_CHECK(model && (n == 0 || (d_in && d_out)), "null argument");

@@ -144,6 +144,14 @@ class NetworkWithInputEncoding:
         _lib.call("nlod_model_encode_fragments", self._h, _stream(x.device), n, _ptr(x), _ptr(out))
         return out
 
+    def mlp_fragments(self, features: torch.Tensor, n: int, out: torch.Tensor | None = None) -> torch.Tensor:
+        """FP16 models: the fused MLP alone on encode_fragments' features of n queries."""
+        if out is None:
+            out = torch.empty((n, self.n_output_dims), dtype=torch.float32, device=features.device)
+        _check_f32(out, "output", (n, self.n_output_dims))
+        _lib.call("nlod_model_mlp_fragments", self._h, _stream(features.device), n, _ptr(features), _ptr(out))
+        return out
+
     def encode(self, x: torch.Tensor, use_inference_params: bool = True) -> torch.Tensor:
         """Encoded features as [encoded_width, n] (SoA)."""
         _check_f32(x, "input")
