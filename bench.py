@@ -35,14 +35,56 @@ ENC_BYTES_PER_QUERY = 16 * 8 * 2 * 2 + 6 * 4 + 48 * 2
 # whole step (encode + MLP): + features read back + 1 x 4 B output
 BYTES_PER_QUERY = ENC_BYTES_PER_QUERY + 48 * 2 + 4
 FLOP_PER_QUERY = 2 * (48 * 64 + 3 * 64 * 64 + 64 * 1)
+# HBM bytes of one grid backward at 2^18 queries of the visibility network,
+# from FETCH_SIZE and WRITE_SIZE passes calibrated by a 256 MiB copy
+# (bench.py --profile-grid-backward); None until profiled
+GRID_BWD_TRAFFIC_BYTES = None
 # HBM bytes of one encoding pass (its two launches) at 2^20 queries, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes calibrated by a 256 MiB copy
 # (bench.py --profile-encode; 420.5 MB fetched + 150.3 MB written)
 ENC_TRAFFIC_BYTES = 570.8e6
 
 
+# L2 view of the encoding pass (its binding roof: the level rows keep the L2
+# channels busy 96% of the pass, 93% of requests hit): 60.2 L2 read requests
+# per query (TCP_TCC_READ_REQ_sum over both launches / queries, profiled at 2^20
+# queries), against MI355X's 8 XCDs x 16 L2 channels x one 128-B request per
+# channel per clock at 2.4 GHz
+ENC_L2_REQ_PER_QUERY = 60.2
+L2_PEAK_REQ_PER_S = 8 * 16 * 2.4e9
+# training step of the visibility network (config 2), per query: the forward
+# MLP FLOPs and twice them for backward (dL/dx and dW); 16 levels x 8 corners x
+# 2 features x 4 B gathered and the same scattered back as FP32 gradients; and
+# per parameter per step Adam's 28 B (gradient, weight, two moments, step count
+# read and written, FP16 copy)
+TRAIN_FLOP_PER_QUERY = 3 * FLOP_PER_QUERY
+TRAIN_GATHER_BYTES_PER_QUERY = 16 * 8 * 2 * 4
+TRAIN_SCATTER_BYTES_PER_QUERY = 16 * 8 * 2 * 4
+ADAM_BYTES_PER_PARAM = 28
+MFMA_FP16_PEAK_TFLOPS = 2500.0
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def running_mean(v, w):
+    c = np.cumsum(np.insert(np.asarray(v, np.float64), 0, 0.0))
+    return (c[w:] - c[:-w]) / w
+
+
+def curve_gap(c32, c16, window=50, tail=100):
+    """Loss-curve agreement of two runs at equal steps: the gap of their means
+    over the last `tail` steps and the largest gap of their `window`-step
+    running means, in percent of the FP32 curve."""
+    r32, r16 = running_mean(c32, window), running_mean(c16, window)
+    gaps = np.abs(r16 - r32) / np.abs(r32) * 100.0
+    m32, m16 = float(np.mean(c32[-tail:])), float(np.mean(c16[-tail:]))
+    return {"fp32_mean_last": m32, "fp16_mean_last": m16, "last_steps": tail,
+            "loss_gap_pct": abs(m16 - m32) / abs(m32) * 100.0,
+            "max_running_gap_pct": float(gaps.max()), "running_window": window,
+            "argmax_running_gap_step": int(np.argmax(gaps)) + window,
+            "fp32_first": float(c32[0]), "fp16_first": float(c16[0])}
 
 
 WAVEFRONT_LODS = 5
@@ -137,11 +179,16 @@ THR_LODS = 5
 THR_SPP = 64
 
 
+THR_POOL = 16  # batches of Monte-Carlo targets the throughput network cycles through
+
+
 def throughput_train_bench(dev, args):
     """Prefiltered-throughput network (hash grid + SH4(wo) + SH4(wi) + LoD
     scalar, MLP 64 x 5 ReLU, 3 exp outputs), RelativeL2, batch 2^18 of valid
     Monte-Carlo targets (64 paths each) of the 200k-triangle blob mesh over a
-    5-level pyramid (256^3 .. 16^3): FP16-MFMA and FP32 plain training steps."""
+    5-level pyramid (256^3 .. 16^3), a pool of 16 batches cycled: FP16-MFMA and
+    FP32 plain training steps, timed, then both trained from the same
+    initialisation for the same steps with every step's loss read back."""
     import torch
     from neural_lod_amd import configs, synthetic
     from neural_lod_amd.labels import Labeler
@@ -154,7 +201,7 @@ def throughput_train_bench(dev, args):
     xs, ts, made, first = [], [], 0, 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    while made < 2 * nb:
+    while made < THR_POOL * nb:
         x, t, v = lb.throughput(gen, seed=200, first=first, spp=THR_SPP, max_depth=3)
         keep = v > 0
         xs.append(x[keep])
@@ -163,37 +210,163 @@ def throughput_train_bench(dev, args):
         first += gen
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
-    x_all, t_all = torch.cat(xs)[:2 * nb].contiguous(), torch.cat(ts)[:2 * nb].contiguous()
-    data = [(x_all[:nb], t_all[:nb]), (x_all[nb:], t_all[nb:])]
-    out = {"batch": nb, "spp": THR_SPP, "lods": THR_LODS,
+    x_all, t_all = torch.cat(xs)[:THR_POOL * nb].contiguous(), torch.cat(ts)[:THR_POOL * nb].contiguous()
+    del xs, ts
+    data = [(x_all[k * nb:(k + 1) * nb], t_all[k * nb:(k + 1) * nb]) for k in range(THR_POOL)]
+    out = {"batch": nb, "spp": THR_SPP, "lods": THR_LODS, "pool_batches": THR_POOL,
            "targets": f"Monte-Carlo voxel-local throughput, {THR_SPP} paths, 200k-triangle blob mesh, "
                       f"{THR_RES}^3 .. {THR_RES >> (THR_LODS - 1)}^3 pyramid, valid samples only",
            "target_gen_ms_per_1k_samples": gen_s * 1e3 / (first / 1024), "valid_fraction": made / first}
     # lr 1e-3: at 1e-2 the exp outputs fall to 0, where RelativeL2 has no gradient
     adam = {"otype": "Adam", "learning_rate": 1e-3, "beta1": 0.9, "beta2": 0.99, "epsilon": 1e-15, "l2_reg": 1e-8}
     out["optimizer"] = adam
+    curves = {}
     for name, prec in (("fp32", FP32), ("fp16", FP16)):
         m = NetworkWithInputEncoding(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"], prec)
         tr = Trainer(m, adam, {"otype": "RelativeL2"}, seed=1337, graphs=True)
         for k in range(max(args.warmup, 2)):
-            tr.training_step(*data[k % 2])
+            tr.training_step(*data[k % THR_POOL])
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
         for k in range(args.train_steps):
-            tr.training_step(*data[k % 2])
+            tr.training_step(*data[k % THR_POOL])
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / args.train_steps
-        st = tr.status()
-        out[name] = {"ms_per_step": ms, "steps_per_s": 1e3 / ms, "final_loss": st["loss"]}
-        log(f"throughput train {name}: {ms:.4f} ms/step, loss {st['loss']:.5f}")
+        out[name] = {"ms_per_step": ms, "steps_per_s": 1e3 / ms}
+        log(f"throughput train {name}: {ms:.4f} ms/step")
+        del tr
+        # the loss curve: a fresh trainer from the same initialisation
+        tr = Trainer(m, adam, {"otype": "RelativeL2"}, seed=1337, graphs=True)
+        losses = np.empty(args.loss_steps)
+        for k in range(args.loss_steps):
+            tr.training_step(*data[k % THR_POOL])
+            losses[k] = tr.status()["loss"]
+        curves[name] = losses
+        out[name]["final_loss"] = float(losses[-1])
+        log(f"throughput {name} loss {losses[0]:.5f} -> {np.mean(losses[-100:]):.5f} over {args.loss_steps} steps")
         del tr, m
     out["speedup_fp16_vs_fp32"] = out["fp32"]["ms_per_step"] / out["fp16"]["ms_per_step"]
-    del lb, mesh
+    out["matched_loss"] = dict(curve_gap(curves["fp32"], curves["fp16"]), steps=args.loss_steps,
+                               data=f"{THR_POOL} batches of 2^18 valid targets, cycled")
+    log(f"throughput matched loss: {out['matched_loss']}")
+    del lb, mesh, data, x_all, t_all
     torch.cuda.empty_cache()
     return out
+
+
+TRACE_RES, TRACE_LODS, TRACE_LOD = 256, 5, 2   # 256^3 voxelization, pyramid to 16^3, traced at 64^3
+TRACE_W, TRACE_H = 1024, 1024                   # 2^20 camera paths per frame
+TRACE_DEPTH = 3
+
+
+def tracer_scene(dev):
+    """The 200k-triangle blob mesh voxelized at 256^3, its LoD pyramid and a
+    camera outside the box looking at its centre."""
+    import torch
+    from neural_lod_amd import synthetic, voxelizer as V
+    from neural_lod_amd import tracer as TR
+    from neural_lod_amd.lod_grid import LodGrid
+    tris = torch.from_numpy(synthetic.blob_mesh(VOX_TRIS, seed=0)).to(dev)
+    table, info = V.voxelize(tris, TRACE_RES)
+    grid = LodGrid(table, info, TRACE_LODS)
+    lo, hi = np.array(info.bbox_min), np.array(info.bbox_max)
+    c, e = (lo + hi) / 2, hi - lo
+    cam = TR.look_at(c + e * np.array([0.35, 0.25, -1.3]), c, (0, 1, 0), 45.0, TRACE_W, TRACE_H)
+    return grid, cam
+
+
+def tracer_bench(dev, args):
+    """The path tracer's frame (voxel_tracing): 2^20 camera paths through the
+    64^3 level of the blob mesh's pyramid, up to 3 bounces, the visibility
+    network (hash grid + SH4, 64 x 4) per voxel segment and the throughput
+    network (64 x 5) per bounce.  FP16: the persistent schedule (one launch,
+    device ray queue) replayed from a hipGraph, and the reference's schedule
+    (compaction rounds with a host count readback each).  FP32 plain: the
+    reference's schedule.  Random-initialised networks (grid tables U[-1, 1])."""
+    import torch
+    from neural_lod_amd import configs
+    from neural_lod_amd import tracer as TR
+    from neural_lod_amd.tcnn import FP16, FP32, NetworkWithInputEncoding
+    from oracle import model as M
+    grid, cam = tracer_scene(dev)
+    n = TRACE_W * TRACE_H
+    res = {"paths": n, "lod_res": TRACE_RES >> TRACE_LOD, "max_depth": TRACE_DEPTH,
+           "scene": f"200k-triangle blob mesh, {TRACE_RES}^3 voxelization, level {TRACE_LOD}"}
+    images = {}
+    for prec_name, prec in (("fp16", FP16), ("fp32", FP32)):
+        nets = []
+        for cfg_, seed in ((configs.VISIBILITY, 5), (configs.THROUGHPUT, 6)):
+            m = NetworkWithInputEncoding(cfg_["n_input_dims"], cfg_["n_output_dims"], cfg_["encoding"],
+                                         cfg_["network"], prec)
+            L_ = M.Layout(cfg_["n_input_dims"], cfg_["n_output_dims"], cfg_["encoding"], cfg_["network"])
+            p_ = torch.from_numpy(M.xavier_uniform(L_, seed, grid_range=1.0)).to(dev)
+            m.set_params(p_)
+            nets.append((m, p_))
+        tr = TR.Tracer(grid, nets[0][0], nets[1][0])
+        schedules = (("persistent", TR.PERSISTENT), ("reference", TR.REFERENCE)) if prec == FP16 else \
+            (("reference", TR.REFERENCE),)
+        for sname, sched in schedules:
+            p = TR.default_params(lod=TRACE_LOD, max_depth=TRACE_DEPTH, schedule=sched)
+            out = tr.render(cam, p)
+            torch.cuda.synchronize()
+            reps = max(3, args.steps // 4) if sched == TR.PERSISTENT else 2
+            graph = None
+            if sched == TR.PERSISTENT:
+                s_ = torch.cuda.Stream()
+                s_.wait_stream(torch.cuda.current_stream())
+                with torch.cuda.stream(s_):
+                    tr.render(cam, p, out=out)
+                torch.cuda.current_stream().wait_stream(s_)
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph):
+                    tr.render(cam, p, out=out)
+                graph.replay()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                if graph is not None:
+                    graph.replay()
+                else:
+                    tr.render(cam, p, out=out)
+            e1.record()
+            torch.cuda.synchronize()
+            wall_ms = (time.perf_counter() - t0) * 1e3 / reps
+            ms = e0.elapsed_time(e1) / reps
+            illum, inf, bnc = (t.clone() for t in out)
+            key = f"{prec_name}_{sname}"
+            images[key] = (illum, inf, bnc)
+            vis_q = int(inf.sum().item())
+            thr_q = int(bnc.sum().item())  # one throughput query per bounce taken (no roulette, exp outputs > 0)
+            res[key] = {"ms_per_frame": ms, "wall_ms_per_frame": wall_ms, "paths_per_s": n / (ms * 1e-3),
+                        "network_queries_per_s": (vis_q + thr_q) / (ms * 1e-3),
+                        "visibility_queries": vis_q, "throughput_queries": thr_q,
+                        "graph": graph is not None}
+            log(f"tracer {key}: {ms:.3f} ms/frame ({wall_ms:.3f} wall), {n / ms / 1e3:.2f} Mpaths/s, "
+                f"{vis_q} vis + {thr_q} thr queries")
+            del graph
+        if prec == FP16:
+            a, b = images["fp16_persistent"], images["fp16_reference"]
+            res["fp16_persistent_equals_reference"] = all(torch.equal(u, v) for u, v in zip(a, b))
+            inf, bnc = a[1].float(), a[2].float()
+            res["path_stats"] = {"mean_visibility_queries": float(inf.mean()),
+                                 "max_visibility_queries": int(inf.max()),
+                                 "bounce_histogram": [int((a[2] == k).sum()) for k in range(TRACE_DEPTH)]}
+        del tr, nets
+    res["speedup_fp16_persistent_vs_fp32_reference"] = res["fp32_reference"]["ms_per_frame"] / \
+        res["fp16_persistent"]["ms_per_frame"]
+    res["speedup_fp16_persistent_vs_fp16_reference"] = res["fp16_reference"]["ms_per_frame"] / \
+        res["fp16_persistent"]["ms_per_frame"]
+    log(f"tracer: {res['speedup_fp16_persistent_vs_fp32_reference']:.2f}x vs FP32 reference schedule, "
+        f"{res['speedup_fp16_persistent_vs_fp16_reference']:.2f}x vs FP16 reference schedule, "
+        f"identical images: {res.get('fp16_persistent_equals_reference')}")
+    del grid
+    torch.cuda.empty_cache()
+    return res
 
 
 def voxelize_bench(dev, args):
@@ -233,18 +406,21 @@ def voxelize_bench(dev, args):
         gpu_table = table.cpu().numpy().view(np.uint32)
         if not args.no_cpu:
             from oracle import oracle as O
-            t0 = time.perf_counter()
-            reps = 0
-            while True:
-                cpu_table, _ = O.voxelize(tris_np, VOX_RES, solid=solid, threads=1)
-                reps += 1
-                if time.perf_counter() - t0 > (4.0 if solid else 2.0):
-                    break
-            dt = (time.perf_counter() - t0) / reps
-            out[mode]["bit_exact_vs_cpu"] = bool(np.array_equal(gpu_table, cpu_table))
-            out[mode]["cpu_baseline"] = {"value": VOX_TRIS / dt, "unit": "triangles/s", "cores": 1, "kind": "port",
-                                         "sample": f"{reps} x the same 200k-triangle mesh at {VOX_RES}^3 "
-                                                   "(C restatement of cpu_voxelizer.cpp, one thread)"}
+            n_threads = int(O.lib().nlo_omp_max_threads())
+            for threads, key in ((1, "cpu_baseline"), (n_threads, "cpu_baseline_omp")):
+                t0 = time.perf_counter()
+                reps = 0
+                while True:
+                    cpu_table, _ = O.voxelize(tris_np, VOX_RES, solid=solid, threads=threads)
+                    reps += 1
+                    if time.perf_counter() - t0 > (4.0 if solid else 2.0):
+                        break
+                dt = (time.perf_counter() - t0) / reps
+                out[mode]["bit_exact_vs_cpu"] = bool(np.array_equal(gpu_table, cpu_table))
+                out[mode][key] = {"value": VOX_TRIS / dt, "unit": "triangles/s", "cores": threads, "kind": "port",
+                                  "sample": f"{reps} x the same 200k-triangle mesh at {VOX_RES}^3 (C restatement of "
+                                            f"cpu_voxelizer.cpp with its omp critical bit writes, {threads} "
+                                            f"OpenMP thread{'s' if threads > 1 else ''})"}
         log(f"voxelize {mode} 200k @ {VOX_RES}^3: {ms * 1e3:.1f} us  {out[mode]}")
     del tris
     for n_big, tag in VOX_TRIS_SWEEP:
@@ -269,6 +445,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-train", action="store_true", help="skip the training-step measurement")
     ap.add_argument("--train-steps", type=int, default=50)
+    ap.add_argument("--loss-steps", type=int, default=2000,
+                    help="steps of the matched-loss curves (FP16 vs FP32 from the same initialisation)")
+    ap.add_argument("--no-tracer", action="store_true", help="skip the path-tracer measurement")
+    ap.add_argument("--profile-grid-backward", action="store_true",
+                    help="only run the grid backward of a 2^18 training batch and a 256 MiB copy (counter profiles)")
+    ap.add_argument("--profile-tracer", action="store_true",
+                    help="only render the path-tracer frames (FP16 persistent) for kernel traces")
+    ap.add_argument("--profile-mlp", action="store_true",
+                    help="only run the FP16 MLP (mlp_fwd_kernel) on pre-encoded 2^20-query features")
+    ap.add_argument("--profile-train", action="store_true",
+                    help="only run FP16 training steps of the visibility network on one 2^18 mesh-label batch")
+    ap.add_argument("--profile-voxelize", action="store_true",
+                    help="only voxelize the 200k- and 20M-triangle blob meshes at 256^3, solid and surface")
     ap.add_argument("--no-wavefront", action="store_true", help="skip the wavefront-batch measurement")
     ap.add_argument("--no-voxelize", action="store_true", help="skip the voxelization measurement")
     ap.add_argument("--profile-encode", action="store_true",
@@ -307,6 +496,88 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log(f"profile-encode: {args.steps} encoding passes of {N_QUERIES} queries, {args.steps} copies of 256 MiB")
+        return
+
+    if args.profile_grid_backward:
+        # the dominant kernel of the FP16 training step alone: grid backward of one
+        # 2^18 batch of mesh-label inputs, then a 256 MiB copy for the byte counters
+        from neural_lod_amd.labels import VisibilityLabeler
+        mesh = torch.from_numpy(synthetic.blob_mesh(VOX_TRIS, seed=0)).to(dev)
+        xb, _ = VisibilityLabeler(mesh, LABEL_RES).sample(TRAIN_BATCH, seed=100)
+        m = NetworkWithInputEncoding(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"], FP16)
+        m.set_params(params)
+        d_enc = torch.randn((m.encoded_width, TRAIN_BATCH), device=dev) * 1e-3
+        d_par = torch.empty(m.n_params, device=dev)
+        for _ in range(args.steps):
+            m.encode_backward(xb, d_enc, d_par)
+        src = torch.empty(1 << 26, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"profile-grid-backward: {args.steps} grid backwards of {TRAIN_BATCH} queries, {args.steps} copies")
+        return
+
+    if args.profile_mlp:
+        m = NetworkWithInputEncoding(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"], FP16)
+        m.set_params(params)
+        feat = m.encode_fragments(x)
+        out = torch.empty((N_QUERIES, 1), device=dev)
+        for _ in range(args.steps):
+            m.mlp_fragments(feat, N_QUERIES, out)
+        torch.cuda.synchronize()
+        log(f"profile-mlp: {args.steps} MLP passes over {N_QUERIES} queries")
+        return
+
+    if args.profile_train:
+        from neural_lod_amd.labels import VisibilityLabeler
+        from neural_lod_amd.tcnn import Trainer
+        mesh = torch.from_numpy(synthetic.blob_mesh(VOX_TRIS, seed=0)).to(dev)
+        xb, yb = VisibilityLabeler(mesh, LABEL_RES).sample(TRAIN_BATCH, seed=100)
+        m = NetworkWithInputEncoding(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"], FP16)
+        adam = {"otype": "Adam", "learning_rate": 1e-2, "beta1": 0.9, "beta2": 0.99, "epsilon": 1e-15,
+                "l2_reg": 1e-8}
+        tr = Trainer(m, adam, {"otype": "BinaryCrossEntropy"}, seed=1337, graphs=False)
+        for _ in range(args.steps):
+            tr.training_step(xb, yb)
+        torch.cuda.synchronize()
+        log(f"profile-train: {args.steps} FP16 training steps of {TRAIN_BATCH} queries (eager launches)")
+        return
+
+    if args.profile_voxelize:
+        import ctypes as C
+        from neural_lod_amd import voxelizer as V
+        stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        for n_tri in (VOX_TRIS, VOX_TRIS_LARGE):
+            tris = torch.from_numpy(synthetic.blob_mesh(n_tri, seed=0)).to(dev)
+            for solid in (True, False):
+                table, info = V.voxelize(tris, VOX_RES, solid=solid)
+                for _ in range(args.steps):
+                    V._lib.call("nlod_voxelize", stream, C.byref(info), C.c_void_p(tris.data_ptr()),
+                                C.c_void_p(table.data_ptr()), V._flags(solid, False, False))
+            torch.cuda.synchronize()
+            del tris
+        log(f"profile-voxelize: {args.steps} + 1 runs each of 200k and 20M triangles, solid then surface")
+        return
+
+    if args.profile_tracer:
+        from neural_lod_amd import tracer as TR
+        grid, cam = tracer_scene(dev)
+        nets = []
+        for cfg_, seed in ((configs.VISIBILITY, 5), (configs.THROUGHPUT, 6)):
+            m = NetworkWithInputEncoding(cfg_["n_input_dims"], cfg_["n_output_dims"], cfg_["encoding"],
+                                         cfg_["network"], FP16)
+            L_ = M.Layout(cfg_["n_input_dims"], cfg_["n_output_dims"], cfg_["encoding"], cfg_["network"])
+            p_ = torch.from_numpy(M.xavier_uniform(L_, seed, grid_range=1.0)).to(dev)
+            m.set_params(p_)
+            nets.append((m, p_))
+        tr = TR.Tracer(grid, nets[0][0], nets[1][0])
+        p = TR.default_params(lod=TRACE_LOD, max_depth=TRACE_DEPTH, schedule=TR.PERSISTENT)
+        out = tr.render(cam, p)
+        for _ in range(args.steps):
+            tr.render(cam, p, out=out)
+        torch.cuda.synchronize()
+        log(f"profile-tracer: {args.steps + 1} persistent frames of {TRACE_W}x{TRACE_H} paths")
         return
 
     models = {}
@@ -355,6 +626,12 @@ def main():
     feat = models["fp16"].encode_fragments(x)
     enc_ms = time_steps(lambda: models["fp16"].encode_fragments(x, feat), args.steps)
     log(f"fp16 encoding pass alone: {enc_ms:.4f} ms")
+    # the fused MLP alone on those features (mlp_fwd_kernel), timed with events
+    mlp_out = torch.empty((N_QUERIES, 1), device=dev)
+    mlp_ms = time_steps(lambda: models["fp16"].mlp_fragments(feat, N_QUERIES, mlp_out), args.steps)
+    torch.cuda.synchronize()
+    assert torch.equal(mlp_out, outs["fp16"]), "encode + MLP alone must equal the inference call"
+    log(f"fp16 MLP alone: {mlp_ms:.4f} ms ({FLOP_PER_QUERY * N_QUERIES / (mlp_ms * 1e-3) / 1e12:.0f} TFLOP/s)")
 
     # accuracy vs the FP64 oracle on a sample of the same batch
     n_chk = 1 << 14
@@ -402,6 +679,57 @@ def main():
             del tr, m
         train["speedup_fp16_vs_fp32"] = train["fp32"]["ms_per_step"] / train["fp16"]["ms_per_step"]
         train["batch"] = nb
+        # matched loss at the configuration's scale: both forms from the same
+        # initialisation, fresh labels every step (sample k of the seeded stream
+        # at step k, identical for both), every step's loss read back
+        curves = {}
+        for name, prec in (("fp32", FP32), ("fp16", FP16)):
+            m = NetworkWithInputEncoding(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"], prec)
+            tr = Trainer(m, adam, {"otype": "BinaryCrossEntropy"}, seed=1337, graphs=True)
+            buf = labeler.sample(nb, seed=300, first=0)
+            losses = np.empty(args.loss_steps)
+            for k in range(args.loss_steps):
+                labeler.sample(nb, seed=300, first=k * nb, out=buf)
+                tr.training_step(*buf)
+                losses[k] = tr.status()["loss"]
+            curves[name] = losses
+            log(f"train {name} loss {losses[0]:.5f} -> {np.mean(losses[-100:]):.5f} over {args.loss_steps} "
+                f"fresh-label steps")
+            del tr, m
+        train["matched_loss"] = dict(curve_gap(curves["fp32"], curves["fp16"]), steps=args.loss_steps,
+                                     data="fresh mesh labels every step (2^18 samples of one seeded stream)")
+        log(f"train matched loss: {train['matched_loss']}")
+        # roofline of the FP16 step: algorithmic bytes and FLOPs per step, and the
+        # dominant kernel, the grid backward (binned scatter of dL/d(features)),
+        # timed alone on the same batch with random feature gradients
+        m16 = NetworkWithInputEncoding(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"], FP16)
+        m16.set_params(params)
+        xb = data[0][0]
+        d_enc = torch.randn((m16.encoded_width, nb), device=dev) * 1e-3
+        d_par = torch.empty(m16.n_params, device=dev)
+        gb_ms = time_steps(lambda: m16.encode_backward(xb, d_enc, d_par), max(args.steps, 20))
+        del m16, d_enc, d_par
+        step_bytes = nb * (TRAIN_GATHER_BYTES_PER_QUERY + TRAIN_SCATTER_BYTES_PER_QUERY) + \
+            ADAM_BYTES_PER_PARAM * L.n_params
+        step_flop = nb * TRAIN_FLOP_PER_QUERY
+        ms16 = train["fp16"]["ms_per_step"]
+        gb_bytes = nb * TRAIN_SCATTER_BYTES_PER_QUERY
+        train["roofline"] = {
+            "bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+            "step_bytes": step_bytes, "step_flop": step_flop,
+            "achieved": step_bytes / (ms16 * 1e-3) / 1e9, "frac": step_bytes / (ms16 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "mfma_tflops": step_flop / (ms16 * 1e-3) / 1e12,
+            "mfma_frac": step_flop / (ms16 * 1e-3) / 1e12 / MFMA_FP16_PEAK_TFLOPS,
+            "dominant_kernel": "grid backward (GridBackward: count, scans, scatter, reduce, apply)",
+            "dominant_ms": gb_ms, "dominant_bytes": gb_bytes,
+            "dominant_achieved": gb_bytes / (gb_ms * 1e-3) / 1e9,
+            "dominant_frac": gb_bytes / (gb_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "dominant_traffic": GRID_BWD_TRAFFIC_BYTES,
+            "dominant_traffic_ratio": None if GRID_BWD_TRAFFIC_BYTES is None else GRID_BWD_TRAFFIC_BYTES / gb_bytes,
+            "basis": "per query: 16 levels x 8 corners x 2 features x 4 B gathered and scattered, 3 x 30,848 "
+                     "MLP FLOP; per parameter 28 B of Adam",
+        }
+        log(f"train roofline: {train['roofline']}")
         train["labels"] = (f"mesh visibility rays: {VOX_TRIS}-triangle blob mesh, origin in one of the "
                            f"{labeler.n_occupied} occupied {LABEL_RES}^3 voxels, direction on the sphere, "
                            "1 = nearest hit inside the voxel")
@@ -429,6 +757,7 @@ def main():
 
     throughput = None if args.no_train else throughput_train_bench(dev, args)
     wavefront = None if args.no_wavefront else wavefront_bench(dev, args)
+    tracer = None if args.no_tracer else tracer_bench(dev, args)
     voxelize = None if args.no_voxelize else voxelize_bench(dev, args)
 
     # CPU baseline: the oracle's FP32 C loop on a bounded sample
@@ -476,12 +805,20 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": ENC_TRAFFIC_BYTES,
                      "kernel": "FP16 encoding pass: encode_lds_kernel<2> (levels 0-1 from LDS) + encode_kernel<2,4,half> (levels 2-15 and SH)",
                      "kernel_ms": enc_ms, "bytes_per_query": ENC_BYTES_PER_QUERY,
-                     "mlp_kernel_ms": fp16["ms"] - enc_ms, "mlp_flop_per_query": FLOP_PER_QUERY,
-                     "mlp_tflops": FLOP_PER_QUERY * N_QUERIES / ((fp16["ms"] - enc_ms) * 1e-3) / 1e12},
+                     # the pass's binding roof is the L2 request rate (tables L2/MALL-resident)
+                     "l2": {"requests_per_query": ENC_L2_REQ_PER_QUERY,
+                            "achieved_req_per_s": ENC_L2_REQ_PER_QUERY * N_QUERIES / (enc_ms * 1e-3),
+                            "peak_req_per_s": L2_PEAK_REQ_PER_S,
+                            "frac": ENC_L2_REQ_PER_QUERY * N_QUERIES / (enc_ms * 1e-3) / L2_PEAK_REQ_PER_S},
+                     "mlp_kernel": "mlp_fwd_kernel (FP16 MFMA 16x16x32, FP32 accumulate), timed alone",
+                     "mlp_kernel_ms": mlp_ms, "mlp_flop_per_query": FLOP_PER_QUERY,
+                     "mlp_tflops": FLOP_PER_QUERY * N_QUERIES / (mlp_ms * 1e-3) / 1e12,
+                     "mlp_frac_mfma_peak": FLOP_PER_QUERY * N_QUERIES / (mlp_ms * 1e-3) / 1e12 / MFMA_FP16_PEAK_TFLOPS},
         "cpu_baseline": cpu,
         "train": train,
         "throughput_train": throughput,
         "wavefront": wavefront,
+        "tracer": tracer,
         "voxelize": voxelize,
     }
     print(json.dumps(line), flush=True)
