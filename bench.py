@@ -37,8 +37,9 @@ BYTES_PER_QUERY = ENC_BYTES_PER_QUERY + 48 * 2 + 4
 FLOP_PER_QUERY = 2 * (48 * 64 + 3 * 64 * 64 + 64 * 1)
 # HBM bytes of one grid backward at 2^18 queries of the visibility network,
 # from FETCH_SIZE and WRITE_SIZE passes calibrated by a 256 MiB copy
-# (bench.py --profile-grid-backward); None until profiled
-GRID_BWD_TRAFFIC_BYTES = None
+# (bench.py --profile-grid-backward; 346.1 MB fetched + 370.6 MB written: the
+# 16-byte edge records are written by the bin pass and read by the reduce)
+GRID_BWD_TRAFFIC_BYTES = 716.7e6
 # HBM bytes of one encoding pass (its two launches) at 2^20 queries, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes calibrated by a 256 MiB copy
 # (bench.py --profile-encode; 420.5 MB fetched + 150.3 MB written)
@@ -180,6 +181,7 @@ THR_SPP = 64
 
 
 THR_POOL = 16  # batches of Monte-Carlo targets the throughput network cycles through
+THR_SEEDS = (1337, 1338, 1339)  # initialisations of the matched-loss runs
 
 
 def throughput_train_bench(dev, args):
@@ -187,12 +189,13 @@ def throughput_train_bench(dev, args):
     scalar, MLP 64 x 5 ReLU, 3 exp outputs), RelativeL2, batch 2^18 of valid
     Monte-Carlo targets (64 paths each) of the 200k-triangle blob mesh over a
     5-level pyramid (256^3 .. 16^3), a pool of 16 batches cycled: FP16-MFMA and
-    FP32 plain training steps, timed, then both trained from the same
-    initialisation for the same steps with every step's loss read back."""
+    FP32 plain training steps, timed, then both trained from each of three
+    initialisations for the same steps, every step's loss read back and the
+    loss over the whole pool taken at the last checkpoints."""
     import torch
     from neural_lod_amd import configs, synthetic
     from neural_lod_amd.labels import Labeler
-    from neural_lod_amd.tcnn import FP16, FP32, NetworkWithInputEncoding, Trainer
+    from neural_lod_amd.tcnn import FP16, FP32, NetworkWithInputEncoding, Trainer, loss_evaluate
     cfg = configs.THROUGHPUT
     nb = TRAIN_BATCH
     mesh = torch.from_numpy(synthetic.blob_mesh(VOX_TRIS, seed=0)).to(dev)
@@ -220,7 +223,17 @@ def throughput_train_bench(dev, args):
     # lr 1e-3: at 1e-2 the exp outputs fall to 0, where RelativeL2 has no gradient
     adam = {"otype": "Adam", "learning_rate": 1e-3, "beta1": 0.9, "beta2": 0.99, "epsilon": 1e-15, "l2_reg": 1e-8}
     out["optimizer"] = adam
-    curves = {}
+    def pool_loss(m):
+        """RelativeL2 of the network over the whole pool (its own forward form)."""
+        tot = 0.0
+        for xb, tb in data:
+            values, _ = loss_evaluate(1, m.inference(xb), tb)
+            tot += float(values.double().sum())
+        return tot / len(data)
+
+    checkpoints = [args.loss_steps - k * args.loss_steps // 8 for k in (2, 1, 0)]
+    curves = {"fp32": [], "fp16": []}
+    pool = {"fp32": [], "fp16": []}
     for name, prec in (("fp32", FP32), ("fp16", FP16)):
         m = NetworkWithInputEncoding(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"], prec)
         tr = Trainer(m, adam, {"otype": "RelativeL2"}, seed=1337, graphs=True)
@@ -238,19 +251,36 @@ def throughput_train_bench(dev, args):
         out[name] = {"ms_per_step": ms, "steps_per_s": 1e3 / ms}
         log(f"throughput train {name}: {ms:.4f} ms/step")
         del tr
-        # the loss curve: a fresh trainer from the same initialisation
-        tr = Trainer(m, adam, {"otype": "RelativeL2"}, seed=1337, graphs=True)
-        losses = np.empty(args.loss_steps)
-        for k in range(args.loss_steps):
-            tr.training_step(*data[k % THR_POOL])
-            losses[k] = tr.status()["loss"]
-        curves[name] = losses
-        out[name]["final_loss"] = float(losses[-1])
-        log(f"throughput {name} loss {losses[0]:.5f} -> {np.mean(losses[-100:]):.5f} over {args.loss_steps} steps")
-        del tr, m
+        # loss curves: fresh trainers from THR_SEEDS' initialisations; the
+        # loss over the whole pool at the last checkpoints of each
+        for seed in THR_SEEDS:
+            tr = Trainer(m, adam, {"otype": "RelativeL2"}, seed=seed, graphs=True)
+            losses = np.empty(args.loss_steps)
+            at = []
+            for k in range(args.loss_steps):
+                tr.training_step(*data[k % THR_POOL])
+                losses[k] = tr.status()["loss"]
+                if k + 1 in checkpoints:
+                    at.append(pool_loss(m))
+            curves[name].append(losses)
+            pool[name].append(float(np.mean(at)))
+            log(f"throughput {name} seed {seed}: loss {losses[0]:.5f} -> {np.mean(losses[-100:]):.5f} over "
+                f"{args.loss_steps} steps, pool loss {pool[name][-1]:.5f}")
+            del tr
+        out[name]["final_loss"] = float(curves[name][0][-1])
+        del m
     out["speedup_fp16_vs_fp32"] = out["fp32"]["ms_per_step"] / out["fp16"]["ms_per_step"]
-    out["matched_loss"] = dict(curve_gap(curves["fp32"], curves["fp16"]), steps=args.loss_steps,
-                               data=f"{THR_POOL} batches of 2^18 valid targets, cycled")
+    p32, p16 = float(np.mean(pool["fp32"])), float(np.mean(pool["fp16"]))
+    out["matched_loss"] = {
+        "statistic": f"loss over all {THR_POOL} pool batches after steps {checkpoints} (mean of the three), "
+                     f"averaged over the initialisations {list(THR_SEEDS)}; the per-step losses cycle through "
+                     f"the pool's batches, whose fit differs, so single curves scatter",
+        "fp32_pool_loss": p32, "fp16_pool_loss": p16,
+        "loss_gap_pct": abs(p16 - p32) / p32 * 100.0,
+        "fp32_per_seed": pool["fp32"], "fp16_per_seed": pool["fp16"],
+        "fp32_seed_spread_pct": (max(pool["fp32"]) - min(pool["fp32"])) / p32 * 100.0,
+        "curves_per_seed": [curve_gap(c32, c16) for c32, c16 in zip(curves["fp32"], curves["fp16"])],
+        "steps": args.loss_steps, "data": f"{THR_POOL} batches of 2^18 valid targets, cycled"}
     log(f"throughput matched loss: {out['matched_loss']}")
     del lb, mesh, data, x_all, t_all
     torch.cuda.empty_cache()
@@ -346,6 +376,8 @@ def tracer_bench(dev, args):
                         "network_queries_per_s": (vis_q + thr_q) / (ms * 1e-3),
                         "visibility_queries": vis_q, "throughput_queries": thr_q,
                         "graph": graph is not None}
+            if sched == TR.PERSISTENT:
+                res[key]["network_evaluations"] = tr.stats()
             log(f"tracer {key}: {ms:.3f} ms/frame ({wall_ms:.3f} wall), {n / ms / 1e3:.2f} Mpaths/s, "
                 f"{vis_q} vis + {thr_q} thr queries")
             del graph
@@ -720,7 +752,7 @@ def main():
             "achieved": step_bytes / (ms16 * 1e-3) / 1e9, "frac": step_bytes / (ms16 * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "mfma_tflops": step_flop / (ms16 * 1e-3) / 1e12,
             "mfma_frac": step_flop / (ms16 * 1e-3) / 1e12 / MFMA_FP16_PEAK_TFLOPS,
-            "dominant_kernel": "grid backward (GridBackward: count, scans, scatter, reduce, apply)",
+            "dominant_kernel": "grid backward (GridBackward: bin, scans, reduce, apply)",
             "dominant_ms": gb_ms, "dominant_bytes": gb_bytes,
             "dominant_achieved": gb_bytes / (gb_ms * 1e-3) / 1e9,
             "dominant_frac": gb_bytes / (gb_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,

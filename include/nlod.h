@@ -529,6 +529,13 @@ int nlod_tracer_destroy(nlod_tracer* tracer);
 int nlod_tracer_render(nlod_tracer* tracer, void* stream, const nlod_camera* camera,
                        const nlod_tracer_params* params, uint32_t sample_idx, uint32_t sample_offset,
                        float* d_illumination, uint32_t* d_inferences, uint32_t* d_bounces);
+/* Counts of the last persistent-schedule frame: stats_out[0] visibility
+ * network evaluations (one per 16-slot tile of a wave), [1] the visibility
+ * queries they served, [2] and [3] the same for the throughput network, [4]
+ * the waves' rounds.  [1] / (16 [0]) is the tile occupancy of the network
+ * evaluations.  Synchronizes the stream; zeros before the first persistent
+ * frame.  No counterpart in the reference. */
+int nlod_tracer_stats(nlod_tracer* tracer, void* stream, uint64_t stats_out[5]);
 /* The schedule nlod_tracer_render would use for these parameters. */
 int nlod_tracer_schedule(const nlod_tracer* tracer, const nlod_tracer_params* params, int* schedule_out);
 

@@ -120,6 +120,7 @@ SIGNATURES: dict[str, list] = {
     "nlod_tracer_destroy": [_vp],
     "nlod_tracer_render": [_vp, _vp, _vp, _vp, _u32, _u32, _vp, _vp, _vp],
     "nlod_tracer_schedule": [_vp, _vp, C.POINTER(_i32)],
+    "nlod_tracer_stats": [_vp, _vp, _vp],
 }
 
 

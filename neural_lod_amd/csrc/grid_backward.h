@@ -23,7 +23,7 @@ class GridBackward {
     void run(hipStream_t s, const float* in, uint32_t N, const float* d_enc, float* grads, bool overwrite);
     // attaches the owner's epoch counter to every buffer (see DevBuf)
     void track(uint64_t* epoch) {
-        for (DevBuf* b : {&d_rows_, &d_bins_, &counts_, &totals_, &bin_start_, &chunk_start_, &meta_, &records_,
+        for (DevBuf* b : {&d_rows_, &d_bins_, &counts_, &run_addr_, &totals_, &chunk_start_, &meta_, &records_,
                           &partial_})
             b->epoch = epoch;
     }
@@ -36,10 +36,13 @@ class GridBackward {
     bool supported_ = false, uploaded_ = false;
     uint32_t F_ = 0, n_rows_ = 0, n_bins_ = 0, corners_per_query_ = 0, n_input_dims_ = 0;
     uint32_t count_rows_ = 0;  // sum over tiles of the repetitions reading the tile
+    uint32_t max_reps_ = 0;    // largest number of segments sharing one table
     struct HostPlan;
     HostPlan* host_ = nullptr;
     DevBuf d_rows_, d_bins_;
-    DevBuf counts_, totals_, bin_start_, chunk_start_, meta_, records_, partial_;
+    // per (tile, repetition, query block): run length (scanned in place into the
+    // run's first record in the tile's order) and the run's first slot
+    DevBuf counts_, run_addr_, totals_, chunk_start_, meta_, records_, partial_;
 };
 
 }  // namespace nlod

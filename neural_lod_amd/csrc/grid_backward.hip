@@ -7,19 +7,30 @@
 // 0.2 T/s, while 64-bit integer LDS atomics reach about 2.4 T/s.  So the same
 // contributions are binned by table tile and summed in LDS in fixed point:
 //
-//   1. count    per (query block, table tile) how many records fall in the
-//               tile; also max |dL/dy| of the pass (sets the fixed-point scale)
-//   2. scan     tile-major exclusive scan -> every block's slot range per tile
-//   3. scatter  records (entry within the tile; F contributions) to their slots,
-//               each record one vector of F + 1 words for F = 2 (12 bytes,
-//               unpadded: the reduce streams them) and of next_pow2(F + 1)
-//               words otherwise, so a record is one store and one load
-//   4. reduce   one workgroup per group of <= kGroupRecs records of one tile:
-//               contributions as 64-bit fixed point (2^S units, S from the
-//               max so that no sum can overflow) added with ds_add_u64; a tile
-//               with one group writes the gradient directly, others a partial
-//   5. apply    tiles with several groups (exact integer sum of the partials)
+//   1. bin      per (query block, level): every x-edge of every query's cell
+//               (the corners x0 and x0 + 1 of one (y, z) row) becomes one
+//               record in its table tile -- or two single-corner records when
+//               the corners lie in different tiles.  The block counts its
+//               records per tile in LDS, gives each tile a run of its own
+//               region of the record buffer, and writes the records there;
+//               per (tile, block) it stores the run's length and address.  It
+//               also raises max |dL/dy| of the pass (sets the fixed-point scale).
+//   2. scan     per tile, exclusive scan of its run lengths over (repetition,
+//               query block) -> the tile's records in one order, cut into
+//               groups of <= kGroupRecs
+//   3. reduce   one workgroup per group: each record's contributions as
+//               64-bit fixed point (2^S units, S from the max so that no sum
+//               can overflow) added with ds_add_u64; a tile with one group
+//               writes the gradient directly, others a partial
+//   4. apply    tiles with several groups (exact integer sum of the partials)
 //               and, on overwrite, tiles without contributions
+//
+// A record is 4 words for F <= 2: key, weight, then F values a_f.  An edge
+// record (key bit 31 set, entries e0 and e1 of the tile in bits 0-12 and
+// 13-25) contributes (1 - fx) a_f to e0 and fx a_f to e1, with a_f = (w_y w_z)
+// g_f and fx the x fraction of the cell; a single record (bit 31 clear, entry
+// e0) contributes weight * a_f.  An edge record moves 16 bytes for two
+// corners where one record per corner moved 12 per corner.
 //
 // Integer sums make the result independent of the order of the additions:
 // the gradient is deterministic.  Contributions below 2^-36 of the pass's
@@ -46,6 +57,8 @@ constexpr uint32_t kGroupRecs = 65536;      // records summed by one reduce work
 constexpr uint32_t kInFlight = 8;           // records loaded per thread before their LDS adds
 constexpr int kFixedHeadroom = 26;          // log2 of the records one entry can receive in a pass, bounded
 constexpr uint32_t kMaxTilesPerRow = 2048;  // LDS counters per (segment, level)
+constexpr uint32_t kMaxRuns = 1024;         // runs of one tile: repetitions x query blocks of a pass
+constexpr uint32_t kEdgeFlag = 1u << 31;
 
 }  // namespace
 
@@ -87,81 +100,37 @@ __device__ __forceinline__ bool load_g(const float* __restrict__ d_enc, uint32_t
     return any;
 }
 
-// Calls emit(idx, weight) for every corner the reference interpolates (grid.h:274-341).
+// The cell of query x at row r as x-edges (grid.h:274-341): for each of the
+// 4 (y, z) rows of the cell, emit(i0, i1, fx, wyz) with the entries of the
+// corners x0 and x0 + 1, the x fraction and the weight w_y w_z; nearest
+// interpolation gives one edge whose two corners coincide (fx = 0, weight 1).
 template <typename Emit>
-__device__ __forceinline__ void corners(const GridBwdRow& r, const float* x, Emit&& emit) {
+__device__ __forceinline__ void edges(const GridBwdRow& r, const float* x, Emit&& emit) {
     float fr[3];
     uint32_t cell[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) grid_pos(x[d], r.scale, r.interp, fr[d], cell[d]);
     if (r.interp == INTERP_NEAREST) {
-        emit(grid_index(r.grid_type, r.hs, r.res, cell[0], cell[1], cell[2]), 1.0f);
+        const uint32_t i = grid_index(r.grid_type, r.hs, r.res, cell[0], cell[1], cell[2]);
+        emit(i, i, 0.0f, 1.0f);
         return;
     }
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        float w = 1.0f;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) w *= ((c >> d) & 1) ? fr[d] : 1.0f - fr[d];
-        emit(grid_index(r.grid_type, r.hs, r.res, cell[0] + (c & 1), cell[1] + ((c >> 1) & 1), cell[2] + ((c >> 2) & 1)),
-             w);
+    for (int p = 0; p < 4; ++p) {
+        const uint32_t dy = p & 1, dz = p >> 1;
+        const float wyz = (dy ? fr[1] : 1.0f - fr[1]) * (dz ? fr[2] : 1.0f - fr[2]);
+        const uint32_t y = cell[1] + dy, z = cell[2] + dz;
+        emit(grid_index(r.grid_type, r.hs, r.res, cell[0], y, z), grid_index(r.grid_type, r.hs, r.res, cell[0] + 1, y, z),
+             fr[0], wyz);
     }
-}
-
-// Pass 1: counts[bin][qblock] and meta[1] = max |dL/dy| bits.  grid (query blocks, rows).
-template <int F>
-__global__ __launch_bounds__(kBlock) void count_kernel(uint32_t N, uint32_t ld, uint32_t n_qblocks, uint32_t in_stride,
-                                                       const GridBwdRow* __restrict__ rows,
-                                                       const float* __restrict__ in, const float* __restrict__ d_enc,
-                                                       uint32_t* __restrict__ counts, uint32_t* __restrict__ meta) {
-    constexpr uint32_t E = kTileFloats / F;
-    __shared__ uint32_t cnt[kMaxTilesPerRow];
-    __shared__ uint32_t wmax[kBlock / 64];
-    const GridBwdRow r = rows[blockIdx.y];
-    for (uint32_t t = threadIdx.x; t < r.n_tiles; t += kBlock) cnt[t] = 0;
-    __syncthreads();
-    // |g| as bits: for non-negative floats the integer order is the float order
-    // (NaN sorts above infinity, so a non-finite value wins the max)
-    uint32_t mx = 0;
-    const uint32_t q0 = blockIdx.x * kQueriesPerBlock;
-    // every query's gradient and position loads first, then the index work
-    float g[kQueriesPerThread][F], x[kQueriesPerThread][3];
-    bool live[kQueriesPerThread];
-#pragma unroll
-    for (uint32_t k = 0; k < kQueriesPerThread; ++k) {
-        const uint32_t q = q0 + k * kBlock + threadIdx.x;
-        live[k] = q < N;
-        const uint32_t qc = live[k] ? q : N - 1;
-        live[k] = load_g<F>(d_enc, r.out_row, ld, qc, g[k]) && live[k];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) x[k][d] = in[(size_t)qc * in_stride + r.in_offset + d];
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < kQueriesPerThread; ++k) {
-        if (!live[k]) continue;
-#pragma unroll
-        for (int f = 0; f < F; ++f) mx = max(mx, __float_as_uint(fabsf(g[k][f])));
-        corners(r, x[k], [&](uint32_t idx, float) { atomicAdd(&cnt[idx / E], 1u); });
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_down((int)mx, o, 64));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t m = 0;
-        for (int w = 0; w < kBlock / 64; ++w) m = max(m, wmax[w]);
-        if (m) atomicMax(&meta[1], m);
-    }
-    for (uint32_t t = threadIdx.x; t < r.n_tiles; t += kBlock)
-        counts[(size_t)(r.count_base + t * r.reps + r.rep) * n_qblocks + blockIdx.x] = cnt[t];
 }
 
 __global__ void reset_meta_kernel(uint32_t* meta) {
     if (threadIdx.x < 4) meta[threadIdx.x] = 0;
 }
 
-// Pass 2a: per tile, exclusive scan over (repetition, query block) in place,
-// and the tile total.  grid: tiles.
+// Pass 2a: per tile, exclusive scan of its run lengths over (repetition,
+// query block) in place, and the tile total.  grid: tiles.
 __global__ __launch_bounds__(kBlock) void scan_rows_kernel(uint32_t n_qblocks, const GridBwdBin* __restrict__ bins,
                                                            uint32_t* __restrict__ counts,
                                                            uint32_t* __restrict__ totals) {
@@ -194,116 +163,94 @@ __global__ __launch_bounds__(kBlock) void scan_rows_kernel(uint32_t n_qblocks, c
     }
 }
 
-// Pass 2b (one block): tile starts and group numbering (a group = up to
-// chunk_recs records of one tile).  meta[0] = groups.
+// Pass 2b (one block): group numbering (a group = up to chunk_recs records
+// of one tile).  meta[0] = groups.
 __device__ __forceinline__ uint32_t groups_of(uint32_t total, uint32_t chunk_recs) {
     return (total + chunk_recs - 1) / chunk_recs;
 }
 
 __global__ __launch_bounds__(kBlock) void scan_bins_kernel(uint32_t n_bins, uint32_t chunk_recs,
                                                            const uint32_t* __restrict__ totals,
-                                                           uint32_t* __restrict__ bin_start,
                                                            uint32_t* __restrict__ group_start,
                                                            uint32_t* __restrict__ meta) {
-    __shared__ uint32_t part_r[kBlock], part_c[kBlock];
+    __shared__ uint32_t part_c[kBlock];
     const uint32_t per = (n_bins + kBlock - 1) / kBlock;
     const uint32_t b0 = min(n_bins, threadIdx.x * per), b1 = min(n_bins, b0 + per);
-    uint32_t sr = 0, sc = 0;
-    for (uint32_t i = b0; i < b1; ++i) {
-        sr += totals[i];
-        sc += groups_of(totals[i], chunk_recs);
-    }
-    part_r[threadIdx.x] = sr;
+    uint32_t sc = 0;
+    for (uint32_t i = b0; i < b1; ++i) sc += groups_of(totals[i], chunk_recs);
     part_c[threadIdx.x] = sc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t ar = 0, ac = 0;
+        uint32_t ac = 0;
         for (int i = 0; i < kBlock; ++i) {
-            const uint32_t vr = part_r[i], vc = part_c[i];
-            part_r[i] = ar;
+            const uint32_t vc = part_c[i];
             part_c[i] = ac;
-            ar += vr;
             ac += vc;
         }
         meta[0] = ac;
         group_start[n_bins] = ac;
     }
     __syncthreads();
-    uint32_t ar = part_r[threadIdx.x], ac = part_c[threadIdx.x];
+    uint32_t ac = part_c[threadIdx.x];
     for (uint32_t i = b0; i < b1; ++i) {
-        bin_start[i] = ar;
         group_start[i] = ac;
-        ar += totals[i];
         ac += groups_of(totals[i], chunk_recs);
     }
 }
 
-// Words per record: the key and F values; padded to a power of two except for
-// F = 2, whose 3-word records load and store as one dwordx3.
+// Words per record: key, weight and F values, padded to a power of two (one
+// dwordx4 for F <= 2).
 template <int F>
 struct RecWords {
-    static constexpr int value = F == 2 ? 3 : F + 1 <= 2 ? 2 : F + 1 <= 4 ? 4 : F + 1 <= 8 ? 8 : 16;
+    static constexpr int value = F + 2 <= 4 ? 4 : F + 2 <= 8 ? 8 : 16;
 };
 
 template <int RW>
 __device__ __forceinline__ void store_rec(uint32_t* __restrict__ dst, const uint32_t (&w)[RW]) {
-    if constexpr (RW == 2) {
-        *reinterpret_cast<uint2*>(dst) = make_uint2(w[0], w[1]);
-    } else if constexpr (RW == 3) {
-        *reinterpret_cast<uint3*>(dst) = make_uint3(w[0], w[1], w[2]);
-    } else {
 #pragma unroll
-        for (int i = 0; i < RW; i += 4)
-            *reinterpret_cast<uint4*>(dst + i) = make_uint4(w[i], w[i + 1], w[i + 2], w[i + 3]);
-    }
+    for (int i = 0; i < RW; i += 4)
+        *reinterpret_cast<uint4*>(dst + i) = make_uint4(w[i], w[i + 1], w[i + 2], w[i + 3]);
 }
 
 template <int RW>
 __device__ __forceinline__ void load_rec(const uint32_t* __restrict__ src, uint32_t (&w)[RW]) {
-    if constexpr (RW == 2) {
-        const uint2 a = *reinterpret_cast<const uint2*>(src);
-        w[0] = a.x;
-        w[1] = a.y;
-    } else if constexpr (RW == 3) {
-        const uint3 a = *reinterpret_cast<const uint3*>(src);
-        w[0] = a.x;
-        w[1] = a.y;
-        w[2] = a.z;
-    } else {
 #pragma unroll
-        for (int i = 0; i < RW; i += 4) {
-            const uint4 a = *reinterpret_cast<const uint4*>(src + i);
-            w[i] = a.x;
-            w[i + 1] = a.y;
-            w[i + 2] = a.z;
-            w[i + 3] = a.w;
-        }
+    for (int i = 0; i < RW; i += 4) {
+        const uint4 a = *reinterpret_cast<const uint4*>(src + i);
+        w[i] = a.x;
+        w[i + 1] = a.y;
+        w[i + 2] = a.z;
+        w[i + 3] = a.w;
     }
 }
 
-// Pass 3: record of slot i = words [i * RW, (i + 1) * RW): entry within the
-// tile, then the F contributions.
+// Records of one (row, query block) live in a region of kQueriesPerBlock x
+// r.corners slots (the most the block can write: every edge split).
+__device__ __forceinline__ size_t region_of(uint32_t row, uint32_t qblock, uint32_t n_qblocks) {
+    return ((size_t)row * n_qblocks + qblock) * kQueriesPerBlock * 8;
+}
+
+// Pass 1: records of (query block, row); run_len / run_addr[count row of
+// (tile, repetition)][query block] = the run's record count and its first
+// slot; meta[1] = max |dL/dy| bits.  grid (query blocks, rows).
 template <int F>
-__global__ __launch_bounds__(kBlock) void scatter_kernel(uint32_t N, uint32_t ld, uint32_t n_qblocks, uint32_t in_stride,
-                                                         const GridBwdRow* __restrict__ rows,
-                                                         const float* __restrict__ in,
-                                                         const float* __restrict__ d_enc,
-                                                         const uint32_t* __restrict__ row_offsets,
-                                                         const uint32_t* __restrict__ bin_start,
-                                                         uint32_t* __restrict__ recs) {
+__global__ __launch_bounds__(kBlock) void bin_kernel(uint32_t N, uint32_t ld, uint32_t n_qblocks, uint32_t in_stride,
+                                                     const GridBwdRow* __restrict__ rows, const float* __restrict__ in,
+                                                     const float* __restrict__ d_enc, uint32_t* __restrict__ run_len,
+                                                     uint32_t* __restrict__ run_addr, uint32_t* __restrict__ recs,
+                                                     uint32_t* __restrict__ meta) {
     constexpr uint32_t E = kTileFloats / F;
     constexpr int RW = RecWords<F>::value;
-    __shared__ uint32_t cursor[kMaxTilesPerRow];
+    __shared__ uint32_t cnt[kMaxTilesPerRow];
+    __shared__ uint32_t part[kBlock];
+    __shared__ uint32_t wmax[kBlock / 64];
     const GridBwdRow r = rows[blockIdx.y];
-    for (uint32_t t = threadIdx.x; t < r.n_tiles; t += kBlock) {
-        const uint32_t b = r.bin_base + t;
-        cursor[t] = bin_start[b] + row_offsets[(size_t)(r.count_base + t * r.reps + r.rep) * n_qblocks + blockIdx.x];
-    }
+    for (uint32_t t = threadIdx.x; t < r.n_tiles; t += kBlock) cnt[t] = 0;
     __syncthreads();
-    const uint32_t q0 = blockIdx.x * kQueriesPerBlock;
     // every query's gradient and position loads first, then the index work
     float gq[kQueriesPerThread][F], xq[kQueriesPerThread][3];
     bool live[kQueriesPerThread];
+    const uint32_t q0 = blockIdx.x * kQueriesPerBlock;
 #pragma unroll
     for (uint32_t k = 0; k < kQueriesPerThread; ++k) {
         const uint32_t q = q0 + k * kBlock + threadIdx.x;
@@ -313,19 +260,83 @@ __global__ __launch_bounds__(kBlock) void scatter_kernel(uint32_t N, uint32_t ld
 #pragma unroll
         for (int d = 0; d < 3; ++d) xq[k][d] = in[(size_t)qc * in_stride + r.in_offset + d];
     }
+    // count the records per tile, and |g| as bits: for non-negative floats the
+    // integer order is the float order (NaN sorts above infinity, so a
+    // non-finite value wins the max)
+    uint32_t mx = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kQueriesPerThread; ++k) {
+        if (!live[k]) continue;
+#pragma unroll
+        for (int f = 0; f < F; ++f) mx = max(mx, __float_as_uint(fabsf(gq[k][f])));
+        edges(r, xq[k], [&](uint32_t i0, uint32_t i1, float, float) {
+            const uint32_t t0 = i0 / E, t1 = i1 / E;
+            atomicAdd(&cnt[t0], 1u);
+            if (t1 != t0) atomicAdd(&cnt[t1], 1u);
+        });
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_down((int)mx, o, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = 0;
+        for (int w = 0; w < kBlock / 64; ++w) m = max(m, wmax[w]);
+        if (m) atomicMax(&meta[1], m);
+    }
+    // exclusive scan of the tile counts (each thread a contiguous run of tiles)
+    // -> the first slot of each tile's run in the block's region; cnt becomes
+    // the cursor of the writes
+    const uint32_t per = (r.n_tiles + kBlock - 1) / kBlock;
+    const uint32_t t_lo = min(r.n_tiles, threadIdx.x * per), t_hi = min(r.n_tiles, t_lo + per);
+    uint32_t sum = 0;
+    for (uint32_t t = t_lo; t < t_hi; ++t) sum += cnt[t];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int i = 0; i < kBlock; ++i) {
+            const uint32_t v = part[i];
+            part[i] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    const size_t region = region_of(blockIdx.y, blockIdx.x, n_qblocks);
+    uint32_t acc = part[threadIdx.x];
+    for (uint32_t t = t_lo; t < t_hi; ++t) {
+        const uint32_t c = cnt[t];
+        const size_t at = (size_t)(r.count_base + t * r.reps + r.rep) * n_qblocks + blockIdx.x;
+        run_len[at] = c;
+        run_addr[at] = (uint32_t)(region + acc);  // slots of one pass stay below 2^32
+        cnt[t] = acc;
+        acc += c;
+    }
+    __syncthreads();
+    // the records
 #pragma unroll
     for (uint32_t k = 0; k < kQueriesPerThread; ++k) {
         if (!live[k]) continue;
         const float* g = gq[k];
-        corners(r, xq[k], [&](uint32_t idx, float w) {
-            const uint32_t slot = atomicAdd(&cursor[idx / E], 1u);
+        edges(r, xq[k], [&](uint32_t i0, uint32_t i1, float fx, float wyz) {
+            const uint32_t t0 = i0 / E, t1 = i1 / E;
             uint32_t rec[RW];
-            rec[0] = idx % E;
 #pragma unroll
-            for (int f = 0; f < F; ++f) rec[1 + f] = __float_as_uint(w * g[f]);
+            for (int f = 0; f < F; ++f) rec[2 + f] = __float_as_uint(wyz * g[f]);
 #pragma unroll
-            for (int i = F + 1; i < RW; ++i) rec[i] = 0u;
-            store_rec<RW>(recs + (size_t)slot * RW, rec);
+            for (int i = F + 2; i < RW; ++i) rec[i] = 0u;
+            if (t0 == t1) {
+                rec[0] = kEdgeFlag | (i0 % E) | ((i1 % E) << 13);
+                rec[1] = __float_as_uint(fx);
+                store_rec<RW>(recs + (region + atomicAdd(&cnt[t0], 1u)) * RW, rec);
+            } else {
+                rec[0] = i0 % E;
+                rec[1] = __float_as_uint(1.0f - fx);
+                store_rec<RW>(recs + (region + atomicAdd(&cnt[t0], 1u)) * RW, rec);
+                rec[0] = i1 % E;
+                rec[1] = __float_as_uint(fx);
+                store_rec<RW>(recs + (region + atomicAdd(&cnt[t1], 1u)) * RW, rec);
+            }
         });
     }
 }
@@ -340,18 +351,28 @@ __device__ __forceinline__ int fixed_shift(uint32_t max_bits) {
 
 __device__ __forceinline__ bool finite_bits(uint32_t b) { return b < 0x7f800000u; }
 
-// Pass 4: group gi of its tile.  grid: upper bound of groups.
+// Pass 3: group gi of its tile.  The tile's records are its runs in
+// (repetition, query block) order: run j holds run_pre[j + 1] - run_pre[j]
+// records from slot run_addr[j].  Each wave walks a contiguous quarter of the
+// group, 64 records (one per lane) at a time; a lane keeps the run its record
+// lies in and moves it forward as its record does (a step of 64 records
+// crosses about one run), so the slot of a record costs one or two LDS reads.
+// grid: upper bound of groups.
 template <int F>
-__global__ __launch_bounds__(kBlock) void reduce_kernel(uint32_t n_bins, const uint32_t* __restrict__ meta,
+__global__ __launch_bounds__(kBlock) void reduce_kernel(uint32_t n_bins, uint32_t n_qblocks,
+                                                        const uint32_t* __restrict__ meta,
                                                         const uint32_t* __restrict__ totals,
-                                                        const uint32_t* __restrict__ bin_start,
                                                         const uint32_t* __restrict__ group_start,
                                                         const GridBwdBin* __restrict__ bins,
+                                                        const uint32_t* __restrict__ run_pre,
+                                                        const uint32_t* __restrict__ run_addr,
                                                         const uint32_t* __restrict__ recs,
                                                         long long* __restrict__ partial, float* __restrict__ grads,
                                                         int overwrite) {
     constexpr int RW = RecWords<F>::value;
+    constexpr uint32_t kWaves = kBlock / 64;
     __shared__ unsigned long long acc[kTileFloats];
+    __shared__ uint32_t s_pre[kMaxRuns + 1], s_addr[kMaxRuns];
     const uint32_t gi = blockIdx.x;
     if (gi >= meta[0]) return;
     // tile of group gi: the last bin whose first group is <= gi
@@ -362,34 +383,71 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(uint32_t n_bins, const u
         else hi = mid;
     }
     const uint32_t b = lo;
-    const uint32_t begin = bin_start[b] + (gi - group_start[b]) * kGroupRecs;
-    const uint32_t end = min(bin_start[b] + totals[b], begin + kGroupRecs);
+    const GridBwdBin bin = bins[b];
+    const uint32_t n_runs = bin.reps * n_qblocks;  // <= kMaxRuns (pass size)
+    const uint32_t total = totals[b];
+    for (uint32_t j = threadIdx.x; j < n_runs; j += kBlock) {
+        s_pre[j] = run_pre[(size_t)bin.count_row * n_qblocks + j];
+        s_addr[j] = run_addr[(size_t)bin.count_row * n_qblocks + j];
+    }
+    if (threadIdx.x == 0) s_pre[n_runs] = total;
+    const uint32_t begin = (gi - group_start[b]) * kGroupRecs;
+    const uint32_t end = min(total, begin + kGroupRecs);
     const uint32_t mbits = meta[1];
     const bool finite = finite_bits(mbits);
     const int S = finite ? fixed_shift(mbits) : 0;
     for (uint32_t i = threadIdx.x; i < kTileFloats; i += kBlock) acc[i] = 0ull;
     __syncthreads();
+    // this wave's part of the group: [w0, w1), a multiple of 64 records long
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t per_wave = (end - begin + 64 * kWaves - 1) / (64 * kWaves) * 64;
+    const uint32_t w0 = min(end, begin + wave * per_wave), w1 = min(end, w0 + per_wave);
+    // the run of record p (p < total): l with s_pre[l] <= p < s_pre[l + 1];
+    // binary search for the lane's first record, then forward steps
+    uint32_t run = 0;
+    {
+        // the lane's first record: the first that load_batch asks for (later ones only grow)
+        const uint32_t p = w0 < w1 ? min(w0 + lane, w1 - 1) : end - 1;
+        uint32_t l = 0, h = n_runs;
+        while (h - l > 1) {
+            const uint32_t m = (l + h) >> 1;
+            if (s_pre[m] <= p) l = m;
+            else h = m;
+        }
+        run = l;
+    }
+    auto slot_of = [&](uint32_t p) -> size_t {
+        while (s_pre[run + 1] <= p) ++run;  // ends: s_pre[n_runs] = total > p
+        return (size_t)s_addr[run] + (p - s_pre[run]);
+    };
     // software-pipelined: the next batch's loads are issued before this
     // batch's LDS adds (lanes past the end load the last record and add 0)
     uint32_t cur[kInFlight][RW] = {}, nxt[kInFlight][RW] = {};  // nxt stays unread after the last batch
     auto load_batch = [&](uint32_t i0, uint32_t (&dst)[kInFlight][RW]) {
+        size_t at[kInFlight];
 #pragma unroll
-        for (uint32_t k = 0; k < kInFlight; ++k) {
-            const uint32_t i = min(i0 + k * kBlock + threadIdx.x, end - 1);
-            load_rec<RW>(recs + (size_t)i * RW, dst[k]);
-        }
+        for (uint32_t k = 0; k < kInFlight; ++k) at[k] = slot_of(min(i0 + k * 64 + lane, w1 - 1));
+#pragma unroll
+        for (uint32_t k = 0; k < kInFlight; ++k) load_rec<RW>(recs + at[k] * RW, dst[k]);
     };
-    if (begin < end) load_batch(begin, cur);
-    for (uint32_t i0 = begin; i0 < end; i0 += kBlock * kInFlight) {
-        const uint32_t i1 = i0 + kBlock * kInFlight;
-        if (i1 < end) load_batch(i1, nxt);
+    auto fixed = [&](float c) { return (unsigned long long)__float2ll_rn(ldexpf(c, S)); };
+    if (w0 < w1) load_batch(w0, cur);
+    for (uint32_t i0 = w0; i0 < w1; i0 += 64 * kInFlight) {
+        const uint32_t i1 = i0 + 64 * kInFlight;
+        if (i1 < w1) load_batch(i1, nxt);
 #pragma unroll
         for (uint32_t k = 0; k < kInFlight; ++k) {
-            const bool valid = i0 + k * kBlock + threadIdx.x < end;
+            const bool valid = i0 + k * 64 + lane < w1;
+            const uint32_t key = cur[k][0];
+            const bool edge = (key & kEdgeFlag) != 0;
+            const uint32_t e0 = key & 0x1fffu, e1 = (key >> 13) & 0x1fffu;
+            const float wt = __uint_as_float(cur[k][1]);
+            const float w0f = edge ? 1.0f - wt : wt;  // an edge's weight word is its x fraction
 #pragma unroll
             for (int f = 0; f < F; ++f) {
-                const long long q = valid ? __float2ll_rn(ldexpf(__uint_as_float(cur[k][1 + f]), S)) : 0ll;
-                atomicAdd(&acc[cur[k][0] * F + f], (unsigned long long)q);
+                const float a = __uint_as_float(cur[k][2 + f]);
+                atomicAdd(&acc[e0 * F + f], valid ? fixed(w0f * a) : 0ull);
+                if (edge) atomicAdd(&acc[e1 * F + f], valid ? fixed(wt * a) : 0ull);
             }
         }
 #pragma unroll
@@ -398,7 +456,6 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(uint32_t n_bins, const u
             for (int j = 0; j < RW; ++j) cur[k][j] = nxt[k][j];
     }
     __syncthreads();
-    const GridBwdBin bin = bins[b];
     const uint32_t n_vals = bin.n_entries * F;
     if (group_start[b + 1] - group_start[b] == 1) {
         // the tile's only group: this is the gradient
@@ -414,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(uint32_t n_bins, const u
     }
 }
 
-// Pass 5: tiles with several groups, and (overwrite) tiles with none.  grid (tiles, kTileFloats / kBlock).
+// Pass 4: tiles with several groups, and (overwrite) tiles with none.  grid (tiles, kTileFloats / kBlock).
 __global__ __launch_bounds__(kBlock) void apply_kernel(const GridBwdBin* __restrict__ bins,
                                                        const uint32_t* __restrict__ group_start,
                                                        const uint32_t* __restrict__ meta,
@@ -528,6 +585,7 @@ GridBackward::GridBackward(const Layout& L) : n_input_dims_(L.n_input_dims) {
     n_rows_ = (uint32_t)plan->rows.size();
     n_bins_ = (uint32_t)plan->bins.size();
     for (const GridBwdRow& r : plan->rows) corners_per_query_ += r.corners;
+    for (const Table& t : tables) max_reps_ = std::max(max_reps_, t.reps);
     host_ = plan.release();
     supported_ = true;
 }
@@ -540,23 +598,26 @@ uint64_t GridBackward::max_groups(uint32_t n) const {
 }
 
 uint32_t GridBackward::queries_per_pass() const {
-    // at most 2^27 records per pass (1.5 GiB at F = 2); an entry then receives at
-    // most 2^27 / (records per query and level) <= 2^kFixedHeadroom contributions
+    // at most 2^27 records per pass (an entry then receives at most 2^27 /
+    // (records per query and level) <= 2^kFixedHeadroom contributions), and at
+    // most kMaxRuns runs (repetitions x query blocks) per tile
     const uint64_t cap = (1ull << 27) / std::max<uint32_t>(corners_per_query_, 1);
-    return (uint32_t)std::max<uint64_t>(kQueriesPerBlock, cap / kQueriesPerBlock * kQueriesPerBlock);
+    const uint64_t runs_cap = (uint64_t)(kMaxRuns / std::max<uint32_t>(max_reps_, 1)) * kQueriesPerBlock;
+    return (uint32_t)std::max<uint64_t>(kQueriesPerBlock,
+                                        std::min(cap, runs_cap) / kQueriesPerBlock * kQueriesPerBlock);
 }
 
 void GridBackward::prepare(uint32_t N) {
     const uint32_t n = std::min(N, queries_per_pass());
     const uint32_t nqb = div_round_up(n, kQueriesPerBlock);
-    const uint64_t max_records = (uint64_t)n * corners_per_query_;
     counts_.ensure(sizeof(uint32_t) * (size_t)count_rows_ * nqb);
+    run_addr_.ensure(sizeof(uint32_t) * (size_t)count_rows_ * nqb);
     totals_.ensure(sizeof(uint32_t) * n_bins_);
-    bin_start_.ensure(sizeof(uint32_t) * n_bins_);
     chunk_start_.ensure(sizeof(uint32_t) * (n_bins_ + 1));
     meta_.ensure(sizeof(uint32_t) * 4);
-    const size_t rec_words = F_ == 2 ? 3 : F_ + 1 <= 2 ? 2 : F_ + 1 <= 4 ? 4 : F_ + 1 <= 8 ? 8 : 16;
-    records_.ensure(sizeof(uint32_t) * (size_t)max_records * rec_words);
+    const size_t rec_words = F_ + 2 <= 4 ? 4 : F_ + 2 <= 8 ? 8 : 16;
+    // every (row, query block) has a region for its largest record count
+    records_.ensure(sizeof(uint32_t) * (size_t)n_rows_ * nqb * kQueriesPerBlock * 8 * rec_words);
     partial_.ensure(sizeof(long long) * (size_t)max_groups(n) * kTileFloats);
 }
 
@@ -586,28 +647,24 @@ void GridBackward::run(hipStream_t s, const float* in, uint32_t N, const float* 
         NLOD_LAUNCH_CHECK();
         by_F(F_, [&](auto Fc) {
             constexpr int F = decltype(Fc)::value;
-            hipLaunchKernelGGL(count_kernel<F>, dim3(nqb, n_rows_), dim3(kBlock), 0, s, n, N, nqb, n_input_dims_,
-                               d_rows_.as<GridBwdRow>(), in_c, denc_c, counts_.as<uint32_t>(), meta_.as<uint32_t>());
+            hipLaunchKernelGGL(bin_kernel<F>, dim3(nqb, n_rows_), dim3(kBlock), 0, s, n, N, nqb, n_input_dims_,
+                               d_rows_.as<GridBwdRow>(), in_c, denc_c, counts_.as<uint32_t>(),
+                               run_addr_.as<uint32_t>(), records_.as<uint32_t>(), meta_.as<uint32_t>());
         });
         NLOD_LAUNCH_CHECK();
         hipLaunchKernelGGL(scan_rows_kernel, dim3(n_bins_), dim3(kBlock), 0, s, nqb, d_bins_.as<GridBwdBin>(),
                            counts_.as<uint32_t>(), totals_.as<uint32_t>());
         NLOD_LAUNCH_CHECK();
         hipLaunchKernelGGL(scan_bins_kernel, dim3(1), dim3(kBlock), 0, s, n_bins_, kGroupRecs,
-                           totals_.as<uint32_t>(), bin_start_.as<uint32_t>(), chunk_start_.as<uint32_t>(),
-                           meta_.as<uint32_t>());
+                           totals_.as<uint32_t>(), chunk_start_.as<uint32_t>(), meta_.as<uint32_t>());
         NLOD_LAUNCH_CHECK();
         const int ow = (overwrite && q0 == 0) ? 1 : 0;
         by_F(F_, [&](auto Fc) {
             constexpr int F = decltype(Fc)::value;
-            hipLaunchKernelGGL(scatter_kernel<F>, dim3(nqb, n_rows_), dim3(kBlock), 0, s, n, N, nqb, n_input_dims_,
-                               d_rows_.as<GridBwdRow>(), in_c, denc_c, counts_.as<uint32_t>(),
-                               bin_start_.as<uint32_t>(), records_.as<uint32_t>());
-            NLOD_LAUNCH_CHECK();
-            hipLaunchKernelGGL(reduce_kernel<F>, dim3((uint32_t)n_groups), dim3(kBlock), 0, s, n_bins_,
-                               meta_.as<uint32_t>(), totals_.as<uint32_t>(), bin_start_.as<uint32_t>(),
-                               chunk_start_.as<uint32_t>(), d_bins_.as<GridBwdBin>(), records_.as<uint32_t>(),
-                               partial_.as<long long>(), grads, ow);
+            hipLaunchKernelGGL(reduce_kernel<F>, dim3((uint32_t)n_groups), dim3(kBlock), 0, s, n_bins_, nqb,
+                               meta_.as<uint32_t>(), totals_.as<uint32_t>(), chunk_start_.as<uint32_t>(),
+                               d_bins_.as<GridBwdBin>(), counts_.as<uint32_t>(), run_addr_.as<uint32_t>(),
+                               records_.as<uint32_t>(), partial_.as<long long>(), grads, ow);
         });
         NLOD_LAUNCH_CHECK();
         hipLaunchKernelGGL(apply_kernel, dim3(n_bins_, kTileFloats / kBlock), dim3(kBlock), 0, s,

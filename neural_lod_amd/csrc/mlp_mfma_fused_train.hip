@@ -5,8 +5,11 @@
 // tiny-cuda-nn fully_fused_mlp.cu:499-731, loss as the reference loss kernels,
 // backward as fully_fused_mlp.cu:150-259,733-833 (no biases).
 //
-// Block = 4 waves (one per SIMD, so a wave may use all 512 registers for its
-// activations and dW accumulators); a round = 4 tiles of 16 queries.  Per round:
+// Block = 8 waves (two per SIMD: while one waits on LDS, a barrier or a load,
+// the other issues; each wave keeps 1/8 of the dW tiles, which keeps it within
+// the 256 registers of two waves per SIMD); a round = 8 tiles of 16 queries.
+// The next round's features and targets are loaded while the current round's
+// backward runs.  Per round:
 //   each wave: features X (B fragments) -> forward through the layers,
 //   keeping every activation as the next layer's B fragment in registers ->
 //   outputs -> loss value (FP64 partial) and dL/dy -> delta_out.
@@ -32,7 +35,7 @@
 namespace nlod {
 namespace {
 
-constexpr int kWaves = 4;  // one wave per SIMD: the whole register file (VGPRs + AGPRs) per wave
+constexpr int kWaves = 8;  // two waves per SIMD
 constexpr int kThreads = kWaves * 64;
 constexpr int kQ = kWaves * 16;       // queries per round
 constexpr int kStride = kQ + 8;       // LDS row stride in halves (16 B aligned rows)
@@ -82,7 +85,29 @@ __device__ __forceinline__ void stage_units(half_t* __restrict__ buf, const half
     }
 }
 
-template <int W, int L>
+// ReLU backward of one chunk on packed halves: the delta G (accumulator tiles
+// lo, hi) rounded to FP16, times the step 1[h > 0] of the activation h.  ReLU
+// outputs are +0, -0 or at least the smallest FP16 subnormal 2^-24, so
+// min(h * 4096 * 4096, 1) is exactly 1 or a zero; the product with a zero
+// step is a zero of either sign (only its sign can differ from rounding
+// G * 0).  Packed FP16 arithmetic: three operations per two values for the
+// step and one for the product.
+__device__ __forceinline__ half8 relu_bwd_pack(const floatx4& lo, const floatx4& hi, const half8& h) {
+    half8 d;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        d[i] = (half_t)lo[i];
+        d[i + 4] = (half_t)hi[i];
+    }
+    const half8 big = {4096, 4096, 4096, 4096, 4096, 4096, 4096, 4096};
+    const half8 one = {1, 1, 1, 1, 1, 1, 1, 1};
+    const half8 step = __builtin_elementwise_min(h * big * big, one);
+    return d * step;
+}
+
+// ACT: ACT_RELU fixed at compile time, or -1 to read the hidden activation
+// from the arguments.
+template <int W, int L, int ACT>
 __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, const half_t* __restrict__ packed,
                                                                const half_t* __restrict__ packed_bwd,
                                                                const half_t* __restrict__ feat,
@@ -106,6 +131,7 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
     __syncthreads();
 
     const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15, wave = threadIdx.x >> 6;
+    const int act = ACT >= 0 ? ACT : a.act;
     const uint32_t kc0 = a.k_padded / 32, kp2 = a.k_padded / 2;
     const uint32_t KT = (a.k_enc + 15) / 16;  // layer-0 dW column tiles
     const uint32_t n_tiles = (a.N + 15) / 16;
@@ -130,6 +156,24 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
     const size_t b_hidden0 = (size_t)MT * 512;
     const size_t b_in = b_hidden0 + (size_t)(L - 1) * MT * KC * 512;
 
+    // features (B fragments, k_padded <= 128: 4 chunks of 32) and targets of
+    // a round's tile; the next round's are loaded during this round's backward
+    half8 xb[4], xn[4];
+    float tg[4], tn[4];
+    auto load_round = [&](uint32_t r, half8(&x)[4], float(&t)[4]) {
+        const uint32_t tile = (r * gridDim.x + blockIdx.x) * kWaves + wave;
+        const bool ok = r < rounds && tile < n_tiles;
+        const uint32_t q = tile * 16 + n;
+#pragma unroll
+        for (uint32_t c = 0; c < 4; ++c) x[c] = ok && c < kc0 ? load_feat_frag(feat, tile, kp2, c, g, n) : half8{};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint32_t oi = 4 * g + i;
+            t[i] = ok && q < a.N && oi < a.n_output ? target[(size_t)q * a.n_output + oi] : 0.0f;
+        }
+    };
+    load_round(0, xb, tg);
+
     for (uint32_t r = 0; r < rounds; ++r) {
         const uint32_t tile = (r * gridDim.x + blockIdx.x) * kWaves + wave;
         const bool tile_ok = tile < n_tiles;
@@ -138,14 +182,12 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
         const int col = wave * 16 + n;
 
         // ---- forward
-        half8 xb[4];  // k_padded <= 128: 4 chunks of 32 features
         floatx4 acc[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) acc[mt] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (uint32_t c = 0; c < 4; ++c) {
             if (c >= kc0) break;
-            xb[c] = tile_ok ? load_feat_frag(feat, tile, kp2, c, g, n) : half8{};
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 const half8 w = *reinterpret_cast<const half8*>(wf + ((size_t)(mt * kc0 + c) * 64 + lane) * 8);
@@ -154,7 +196,7 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
         }
         half8 h[L][KC];
 #pragma unroll
-        for (int c = 0; c < KC; ++c) h[0][c] = pack_act(acc[2 * c], acc[2 * c + 1], a.act);
+        for (int c = 0; c < KC; ++c) h[0][c] = pack_act(acc[2 * c], acc[2 * c + 1], act);
 #pragma unroll
         for (int l = 1; l < L; ++l) {
             const half_t* fr = wf + f_hidden0 + (size_t)(l - 1) * MT * KC * 512;
@@ -169,7 +211,7 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
                 acc[mt] = s;
             }
 #pragma unroll
-            for (int c = 0; c < KC; ++c) h[l][c] = pack_act(acc[2 * c], acc[2 * c + 1], a.act);
+            for (int c = 0; c < KC; ++c) h[l][c] = pack_act(acc[2 * c], acc[2 * c + 1], act);
         }
         floatx4 o = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -188,13 +230,15 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
                 const float p = act_fwd(a.out_act, o[i]);
                 double v;
                 float gr;
-                loss_elem(a.loss_type, p, target[(size_t)q * a.n_output + oi], n_total, scale, v, gr);
+                loss_elem(a.loss_type, p, tg[i], n_total, scale, v, gr);
                 lsum += v;
                 d = gr * act_bwd(a.out_act, p);
             }
             bo[i] = (half_t)d;
             bo[i + 4] = (half_t)0.0f;
         }
+
+        load_round(r + 1, xn, tn);
 
         // ---- output layer: dW_out = delta_out . H_{L-1}^T
 #pragma unroll
@@ -216,14 +260,18 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
         for (int l = L - 1; l >= 0; --l) {
 #pragma unroll
             for (int c = 0; c < KC; ++c) {
-                float y[8], d[8];
+                if constexpr (ACT == ACT_RELU) {
+                    dl[c] = relu_bwd_pack(acc[2 * c], acc[2 * c + 1], h[l][c]);
+                } else {
+                    float y[8], d[8];
 #pragma unroll
-                for (int i = 0; i < 8; ++i) y[i] = (float)h[l][c][i];
-                act_bwd_n<8>(a.act, y, d);
+                    for (int i = 0; i < 8; ++i) y[i] = (float)h[l][c][i];
+                    act_bwd_n<8>(act, y, d);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    dl[c][i] = (half_t)(acc[2 * c][i] * d[i]);
-                    dl[c][i + 4] = (half_t)(acc[2 * c + 1][i] * d[i + 4]);
+                    for (int i = 0; i < 4; ++i) {
+                        dl[c][i] = (half_t)(acc[2 * c][i] * d[i]);
+                        dl[c][i + 4] = (half_t)(acc[2 * c + 1][i] * d[i + 4]);
+                    }
                 }
             }
             stage_units<KC>(bufA, dl, col);
@@ -288,6 +336,10 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
                 }
             }
         }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) xb[c] = xn[c];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) tg[i] = tn[i];
     }
 
     // ---- per-block results: loss partial and dW partial (row-major matrices)
@@ -419,7 +471,7 @@ void mfma_fused_train(hipStream_t stream, const Layout& L, const void* packed, c
         if constexpr (W == 64 && LL > 5) {
             fail(NLOD_ERR_UNSUPPORTED, "fused training supports at most 5 hidden layers at width 64");
         } else {
-            auto kern = fused_train_kernel<W, LL>;
+            auto kern = a.act == ACT_RELU ? fused_train_kernel<W, LL, ACT_RELU> : fused_train_kernel<W, LL, -1>;
             (void)occupancy_blocks((const void*)kern, kThreads, smem);  // raises the dynamic LDS limit
             hipLaunchKernelGGL(kern, dim3(blocks), dim3(kThreads), smem, stream, a,
                                static_cast<const half_t*>(packed), static_cast<const half_t*>(packed_bwd),

@@ -611,6 +611,9 @@ __global__ __launch_bounds__(kTraceBlock) void trace_persistent_kernel(TracerArg
     uint32_t k = 0, b = 0;        // visibility round and bounce of the slot's ray
     uint32_t next = 0, end = 0;   // rays taken from the queue and not yet started
     bool drained = false;
+    // wave-uniform tallies for the frame's statistics: network evaluations of
+    // the tile and the slots they served, per network, and the wave's rounds
+    uint32_t n_vis_tiles = 0, n_vis_q = 0, n_thr_tiles = 0, n_thr_q = 0, n_rounds = 0;
     for (;;) {
         // refill free slots in order from the wave's block of the queue
         uint32_t free_slots = slot_mask(__ballot(!live));
@@ -639,6 +642,7 @@ __global__ __launch_bounds__(kTraceBlock) void trace_persistent_kernel(TracerArg
             free_slots = slot_mask(__ballot(!live));
         }
         if (slot_mask(__ballot(live)) == 0) break;  // the queue is empty and every slot is done
+        ++n_rounds;
 
         if (live && want == kWantMarch) {
             if (!march(a, r)) {
@@ -656,6 +660,8 @@ __global__ __launch_bounds__(kTraceBlock) void trace_persistent_kernel(TracerArg
         const uint32_t n_thr = (uint32_t)__popc(slot_mask(__ballot(live && want == kWantThr)));
         if (n_vis == 0 && n_thr == 0) continue;
         if (n_thr >= n_vis) {
+            ++n_thr_tiles;
+            n_thr_q += n_thr;
             float in[10];
             const bool mine = live && want == kWantThr;
             if (mine) throughput_inputs(a, r, in);
@@ -677,6 +683,8 @@ __global__ __launch_bounds__(kTraceBlock) void trace_persistent_kernel(TracerArg
                 }
             }
         } else {
+            ++n_vis_tiles;
+            n_vis_q += n_vis;
             float in[6];
             const bool mine = live && want == kWantVis;
             if (mine) visibility_inputs(a, r, in);
@@ -703,10 +711,19 @@ __global__ __launch_bounds__(kTraceBlock) void trace_persistent_kernel(TracerArg
             }
         }
     }
+    if (lane == 0) {
+        unsigned long long* st = reinterpret_cast<unsigned long long*>(queue + 16);
+        atomicAdd(st + 0, (unsigned long long)n_vis_tiles);
+        atomicAdd(st + 1, (unsigned long long)n_vis_q);
+        atomicAdd(st + 2, (unsigned long long)n_thr_tiles);
+        atomicAdd(st + 3, (unsigned long long)n_thr_q);
+        atomicAdd(st + 4, (unsigned long long)n_rounds);
+    }
 }
 
+// queue[0]: next ray; queue[16 ..]: the frame's statistics (uint64 each)
 __global__ void reset_queue_kernel(uint32_t* q) {
-    if (threadIdx.x == 0) q[0] = 0;
+    if (threadIdx.x < 64) q[threadIdx.x] = 0;
 }
 
 }  // namespace
@@ -836,6 +853,14 @@ struct Tracer {
         NLOD_LAUNCH_CHECK();
     }
 
+    // statistics of the last persistent frame (synchronizes the stream)
+    void stats(hipStream_t s, uint64_t* out) {
+        std::memset(out, 0, 5 * sizeof(uint64_t));
+        if (!queue.p) return;
+        NLOD_HIP(hipMemcpyAsync(out, queue.as<uint32_t>() + 16, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        NLOD_HIP(hipStreamSynchronize(s));
+    }
+
     uint32_t read_count(hipStream_t s, int which) {
         NLOD_HIP(hipMemcpyAsync(host_counts + which, counters.as<uint32_t>() + which, sizeof(uint32_t),
                                 hipMemcpyDeviceToHost, s));
@@ -956,6 +981,13 @@ int nlod_tracer_render(nlod_tracer* tracer, void* stream, const nlod_camera* cam
         NLOD_CHECK(tracer && camera && params && d_illumination, "null argument");
         tracer->t->render(nlod::as_stream(stream), *camera, *params, sample_idx, sample_offset, d_illumination,
                           d_inferences, d_bounces);
+    });
+}
+
+int nlod_tracer_stats(nlod_tracer* tracer, void* stream, uint64_t* stats_out) {
+    return nlod::guarded([&] {
+        NLOD_CHECK(tracer && stats_out, "null argument");
+        tracer->t->stats(nlod::as_stream(stream), stats_out);
     });
 }
 

@@ -106,6 +106,16 @@ class Tracer:
         _lib.call("nlod_tracer_schedule", self._h, C.byref(params), C.byref(out))
         return out.value
 
+    def stats(self) -> dict:
+        """Network evaluations of the last persistent frame and their tile occupancy."""
+        out = (C.c_uint64 * 5)()
+        _lib.call("nlod_tracer_stats", self._h, C.c_void_p(torch.cuda.current_stream(self.grid.device).cuda_stream),
+                  out)
+        vt, vq, tt, tq, rounds = (int(v) for v in out)
+        return {"visibility_tiles": vt, "visibility_queries": vq, "throughput_tiles": tt, "throughput_queries": tq,
+                "wave_rounds": rounds,
+                "tile_occupancy": (vq + tq) / (16.0 * (vt + tt)) if vt + tt else 0.0}
+
     def render(self, camera: Camera, params: TracerParams, sample_idx: int = 0, sample_offset: int = 0,
                out=None, with_stats: bool = True):
         """(illumination float32 [h*w, 3], visibility queries int32 [h*w],

@@ -30,7 +30,7 @@ RECOMPUTE_ABOVE = 1 << 20
 def _inputs(cfg, n, seed):
     if cfg["n_input_dims"] == 6:
         return synthetic.visibility_queries(n, seed)
-    return synthetic.throughput_queries(n, seed=seed)
+    return np.ascontiguousarray(synthetic.throughput_queries(n, seed=seed)[:, :cfg["n_input_dims"]])
 
 
 def _model(cfg, precision):
@@ -88,7 +88,8 @@ def _features(m, L, xt):
 
 
 def _fp16_restatement(cfg, L, p, X, dy):
-    """Forward + backward of the MLP rounding to FP16 where the MFMA kernels do."""
+    """Forward + backward of the MLP rounding to FP16 where the MFMA kernels do.
+    ``dy``: dL/dy [n][n_output], or a function of the outputs y giving it."""
     net = cfg["network"]
     hid = net.get("activation", "ReLU").lower()
     out_act = net.get("output_activation", "None").lower()
@@ -102,7 +103,9 @@ def _fp16_restatement(cfg, L, p, X, dy):
     hs = [X]
     for Wm in mats[:-1]:
         hs.append(_f16(_act(hid, hs[-1] @ Wm.T)))
-    y = _act(out_act, hs[-1] @ mats[-1].T)[:, :dy.shape[1]]
+    y = _act(out_act, hs[-1] @ mats[-1].T)[:, :cfg["n_output_dims"]]
+    if callable(dy):
+        dy = dy(y)
     d_out = np.zeros((X.shape[0], n.n_output_padded))
     d_out[:, :dy.shape[1]] = _f16(dy * _act_bwd(out_act, y))
     deltas = [d_out]
@@ -114,7 +117,7 @@ def _fp16_restatement(cfg, L, p, X, dy):
     return np.concatenate([g.ravel() for g in grads]), d_enc
 
 
-@pytest.mark.parametrize("name", ["visibility", "throughput", "reference_visibility"])
+@pytest.mark.parametrize("name", ["visibility", "throughput", "reference_visibility", "reference_throughput"])
 def test_backward_fp16(cuda, name):
     cfg = configs.ALL[name]
     print(f"case backward fp16 {name}", flush=True)
@@ -133,7 +136,7 @@ def test_backward_fp16(cuda, name):
     xt = torch.from_numpy(x).cuda()
     out, ctx = m.forward(xt)
     err = np.abs(out.cpu().numpy() - ref_out).max()
-    assert err <= 2e-3, f"forward max abs err {err:.3e}"
+    assert err <= 2e-3 * max(1.0, float(np.abs(ref_out).max())), f"forward max abs err {err:.3e}"
     g, _ = m.backward(ctx, xt, torch.from_numpy(dy).cuda())
     g = g.cpu().numpy().astype(np.float64)
     # FP16 restatement with the kernel's own features; grid part through the oracle's scatter
@@ -152,6 +155,53 @@ def test_backward_fp16(cuda, name):
         print(f"  {blk}: rel L2 err vs FP16 restatement {rel_emu:.2e}, vs FP64 oracle {rel_ref:.2e}", flush=True)
         if rel_emu > 2e-3 or rel_ref > 0.1:
             failures.append((blk, rel_emu, rel_ref))
+    assert not failures, f"{name}: {failures}"
+
+
+@pytest.mark.parametrize("name", ["visibility", "throughput", "reference_throughput"])
+def test_training_step_gradients_vs_fp16_restatement(cuda, name):
+    """One FP16 training step's gradients -- from the fused forward + loss +
+    backward + dW kernel where the layout allows it (visibility, throughput),
+    from the separate kernels for the width-128 reference_throughput -- against
+    the FP16 restatement fed with the step's own FP16 features and the
+    oracle's loss gradient at the step's loss scale (the trainer keeps its
+    gradients scaled): ||g - g_emu||_2 <= 2e-3 ||g_emu||_2 per block, as above."""
+    from neural_lod_amd.tcnn import Trainer
+    cfg = configs.ALL[name]
+    print(f"case training step gradients {name}", flush=True)
+    m = _model(cfg, 1)
+    L = M.Layout(cfg["n_input_dims"], cfg["n_output_dims"], cfg["encoding"], cfg["network"])
+    p = M.xavier_uniform(L, 41, grid_range=0.5)
+    n = 3000
+    x = _inputs(cfg, n, 42)
+    if cfg["n_input_dims"] == 6:
+        t, kind = synthetic.sphere_visibility_labels(x), "BinaryCrossEntropy"
+    else:
+        t = np.stack([0.3 + 0.2 * np.sin(4 * x[:, 0]), 0.5 + 0.1 * x[:, 1], 0.2 + 0.3 * x[:, 2]], 1)
+        t, kind = t.astype(np.float32), "RelativeL2"
+    scale = 1024.0
+    adam = {"otype": "Adam", "learning_rate": 1e-3, "beta1": 0.9, "beta2": 0.99, "epsilon": 1e-15, "l2_reg": 1e-8}
+    tr = Trainer(m, adam, {"otype": kind}, seed=5, loss_scale_config={"initial_scale": scale, "dynamic": False})
+    tr.set_params(torch.from_numpy(p.astype(np.float32)).cuda())
+    xt = torch.from_numpy(x).cuda()
+    feats = _features(m, L, xt)  # the tables of the step (read before the optimizer moves them)
+    tr.training_step(xt, torch.from_numpy(t).cuda())
+    g = tr.gradients().cpu().numpy().astype(np.float64)
+    p32 = p.astype(np.float32).astype(np.float64)
+    g_net, d_enc = _fp16_restatement(cfg, L, p32, feats,
+                                     lambda y: M.loss(kind, y, t, cfg["n_output_dims"], scale)[1])
+    g_emu = np.zeros(L.n_params)
+    g_emu[:L.n_net_params] = g_net
+    d_enc_full = np.zeros((n, L.enc_width))
+    d_enc_full[:, :d_enc.shape[1]] = d_enc
+    M._lib().nlo_encode_backward(L.segs, len(L.segs), L.enc_width, M._fp(x), x.shape[1], n, M._dp(d_enc_full),
+                                 M._dp(g_emu))
+    failures = []
+    for blk, sl in _blocks(L):
+        rel = np.linalg.norm(g[sl] - g_emu[sl]) / max(np.linalg.norm(g_emu[sl]), 1e-30)
+        print(f"  {blk}: rel L2 err vs FP16 restatement {rel:.2e}", flush=True)
+        if not rel <= 2e-3:
+            failures.append((blk, rel))
     assert not failures, f"{name}: {failures}"
 
 

@@ -97,12 +97,20 @@ def test_inference_fp32(cuda, name):
         _fp32_close(out, ref, f"inference {name} n={n}")
 
 
-@pytest.mark.parametrize("name", ["visibility", "throughput", "reference_visibility", "dense_smoothstep", "hash_f8"])
+FP16_CASES = ["visibility", "throughput", "reference_visibility", "reference_throughput", "dense_smoothstep",
+              "hash_f8"]
+
+
+@pytest.mark.parametrize("name", FP16_CASES)
 @pytest.mark.parametrize("grid_range", [1e-4, 1.0])
 def test_inference_fp16_mfma(cuda, name, grid_range):
-    """FP16 MFMA form vs the FP64 oracle.  dense_smoothstep (F = 4, all levels
-    small enough for the LDS pass) and hash_f8 (F = 8, 64 KB levels: one in the
-    LDS pass, the rest in the per-level pass) cover the other feature widths."""
+    """FP16 MFMA form vs the FP64 oracle: max abs error <= 2e-3 of the output
+    scale (max(1, max |y|): outputs without a bounded activation grow past 1).
+    reference_throughput is the reference's production throughput network
+    (two SH degree 8 blocks, K = 160, width 128, linear outputs).
+    dense_smoothstep (F = 4, all levels small enough for the LDS pass) and
+    hash_f8 (F = 8, 64 KB levels: one in the LDS pass, the rest in the
+    per-level pass) cover the other feature widths."""
     cfg = ALL_CFGS[name]
     print(f"case inference fp16 {name} grid_range={grid_range}", flush=True)
     m = _model(cfg, precision=1)
@@ -113,11 +121,12 @@ def test_inference_fp16_mfma(cuda, name, grid_range):
         m.set_params(torch.from_numpy(p).cuda())
         out = m.inference(torch.from_numpy(x).cuda()).cpu().numpy()
         err = np.abs(out - ref).max()
-        print(f"  n={n} max abs err {err:.3e}", flush=True)
-        assert err <= 2e-3, f"FP16 MFMA {name} n={n}: max abs error {err:.3e} > 2e-3"
+        tol = 2e-3 * max(1.0, float(np.abs(ref).max()))
+        print(f"  n={n} max abs err {err:.3e} (tolerance {tol:.3e})", flush=True)
+        assert err <= tol, f"FP16 MFMA {name} n={n}: max abs error {err:.3e} > {tol:.3e}"
 
 
-@pytest.mark.parametrize("name", ["visibility", "throughput", "reference_visibility", "dense_smoothstep", "hash_f8"])
+@pytest.mark.parametrize("name", FP16_CASES)
 def test_encoding_pass_features_fp16(cuda, name):
     """The FP16 encoding pass feature by feature (coarse levels from the LDS
     pass, the other levels from the per-level rows, SH / identity / pad rows
@@ -139,7 +148,12 @@ def test_encoding_pass_features_fp16(cuda, name):
     ref = M.encode(L, p16, x)
     w = ref.shape[1]
     err = np.abs(got[:n, :w] - ref)
-    bound = 2.0 ** -10 * np.abs(ref) + 1e-6  # FP16 rounding of the result, FP32 interpolation
+    # FP16 rounding of the result, plus the FP32 arithmetic before it: interpolation
+    # (1e-6), or the degree-8 SH polynomials, whose terms reach ~30 before they
+    # cancel (an FP32 evaluation of them on the CPU is off by up to 4e-6)
+    sh8 = any(e.get("otype") == "SphericalHarmonics" and e.get("degree", 4) > 4
+              for e in cfg["encoding"].get("nested", [cfg["encoding"]]))
+    bound = 2.0 ** -10 * np.abs(ref) + (8e-6 if sh8 else 1e-6)
     assert np.all(err <= bound), f"{name}: {np.sum(err > bound)} features outside FP16 rounding, max {err.max():.3e}"
     assert not np.any(got[:n, w:]), f"{name}: K padding rows are not 0"
     assert not np.any(got[n:]), f"{name}: padding queries of the last tile are not 0"

@@ -115,7 +115,13 @@ def test_persistent_equals_reference_schedule(scene, name):
     pp = TR.default_params(schedule=TR.PERSISTENT, **kw)
     assert tr.schedule(TR.default_params(**kw)) == TR.PERSISTENT
     for sample in (0, 5):
-        _assert_same(_render(tr, cam, pp, sample), _render(tr, cam, pr, sample), f"{name}, sample {sample}")
+        got = _render(tr, cam, pp, sample)
+        st = tr.stats()
+        _assert_same(got, _render(tr, cam, pr, sample), f"{name}, sample {sample}")
+        # the frame's statistics count every visibility query once, in 16-slot tiles
+        assert st["visibility_queries"] == int(got[1].astype(np.int64).sum())
+        assert 16 * st["visibility_tiles"] >= st["visibility_queries"] >= st["visibility_tiles"] > 0
+        assert 16 * st["throughput_tiles"] >= st["throughput_queries"] >= st["throughput_tiles"]
 
 
 def test_options_equal_oracle(scene):

@@ -282,21 +282,29 @@ def test_snapshot_roundtrip(cuda):
 def test_fp16_loss_curve_within_2pct_of_fp32(cuda):
     """Visibility network (16-level hash grid + SH4, MLP 64x4) trained on
     analytic sphere-occlusion labels: FP16-MFMA and FP32 plain forms from the
-    same initial parameters; mean loss of the last 50 of 300 steps within 2%."""
+    same initial parameters, for three initialisations; the mean over them of
+    the loss of the last 50 of 300 steps agrees within 2%.  One initialisation
+    alone is not enough at this size: two runs of one form from different
+    seeds end up to ~2% apart, so single pairs measure that spread, not the
+    arithmetic."""
     cfg = configs.VISIBILITY
-    curves = {}
-    for prec in (0, 1):
-        tr = _trainer(cfg, prec, seed=1337)
-        losses = []
-        for k in range(300):
-            x = synthetic.visibility_queries(1 << 14, 1000 + k)
-            t = synthetic.sphere_visibility_labels(x)
-            tr.training_step(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda())
-            losses.append(tr.loss())
-        curves[prec] = np.array(losses)
-        print(prec, curves[prec][::30])
-    l32, l16 = curves[0][-50:].mean(), curves[1][-50:].mean()
-    assert curves[0][-50:].mean() < 0.6 * curves[0][:10].mean(), "the task must be learnable"
+    batches = []
+    for k in range(300):
+        x = synthetic.visibility_queries(1 << 14, 1000 + k)
+        batches.append((torch.from_numpy(x).cuda(), torch.from_numpy(synthetic.sphere_visibility_labels(x)).cuda()))
+    finals = {0: [], 1: []}
+    for seed in (1337, 7, 2024):
+        for prec in (0, 1):
+            tr = _trainer(cfg, prec, seed=seed)
+            losses = []
+            for x, t in batches:
+                tr.training_step(x, t)
+                losses.append(tr.loss())
+            losses = np.array(losses)
+            assert losses[-50:].mean() < 0.6 * losses[:10].mean(), "the task must be learnable"
+            finals[prec].append(losses[-50:].mean())
+            print(f"seed {seed} precision {prec}: {losses[::60]} -> {finals[prec][-1]:.5f}", flush=True)
+    l32, l16 = np.mean(finals[0]), np.mean(finals[1])
     assert abs(l16 - l32) <= 0.02 * l32, f"FP16 {l16:.5f} vs FP32 {l32:.5f}"
 
 
