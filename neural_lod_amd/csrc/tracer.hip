@@ -226,9 +226,10 @@ __device__ __forceinline__ float hg_pdf(float cos_wo_wi, float g) {
 }
 
 // ------------------------------------------------------------------ stages
-__device__ __forceinline__ bool occupied(const TracerArgs& a, int x, int y, int z) {
+// bits: the level's occupancy words (global memory, or the block's LDS copy)
+__device__ __forceinline__ bool occupied(const TracerArgs& a, const uint32_t* bits, int x, int y, int z) {
     const uint32_t i = (uint32_t)x + (uint32_t)y * a.R + (uint32_t)z * a.R * a.R;
-    return (a.bits[i >> 5] >> (31u - (i & 31u))) & 1u;
+    return (bits[i >> 5] >> (31u - (i & 31u))) & 1u;
 }
 
 // camera_raygen (rptr/cuda/wavefront_neural_pt.cu:9-49)
@@ -262,8 +263,9 @@ __device__ __forceinline__ void raygen(const TracerArgs& a, uint32_t i, TRay& r)
 }
 
 // raymarch_impl (wavefront_neural_pt.cuh:89-141): first occupied voxel other
-// than the previous one from the ray's minimum time; false = escaped.
-__device__ __forceinline__ bool march(const TracerArgs& a, TRay& r) {
+// than the previous one from the ray's minimum time; false = escaped.  The
+// occupancy words come from `bits` (a.bits or an LDS copy of them).
+__device__ __forceinline__ bool march(const TracerArgs& a, const uint32_t* bits, TRay& r) {
     const int R = (int)a.R;
     const float Rf = (float)a.R;
     float inv[3];
@@ -295,7 +297,7 @@ __device__ __forceinline__ bool march(const TracerArgs& a, TRay& r) {
     }
     const int max_steps = 3 * R + 3;
     for (int step = 0; step < max_steps; ++step) {
-        if (occupied(a, v[0], v[1], v[2]) && !(v[0] == r.vox[0] && v[1] == r.vox[1] && v[2] == r.vox[2])) {
+        if (occupied(a, bits, v[0], v[1], v[2]) && !(v[0] == r.vox[0] && v[1] == r.vox[1] && v[2] == r.vox[2])) {
             // intersect_aabb_min_max with the voxel's box
             float nearT = -1e20f, farT = 1e20f;
             bool hit = true;
@@ -510,7 +512,7 @@ __global__ __launch_bounds__(kStageThreads) void march_kernel(TracerArgs a, uint
     if (i >= n) return;
     TRay r = rays[i];
     if (r.state != kActive) return;
-    if (!march(a, r)) r.state = kEscaped;
+    if (!march(a, a.bits, r)) r.state = kEscaped;
     rays[i] = r;
 }
 
@@ -580,11 +582,15 @@ __global__ __launch_bounds__(kStageThreads) void bounce_compact_kernel(TracerArg
 constexpr int kTraceBlock = 512;
 constexpr int kTraceWaves = kTraceBlock / 64;
 constexpr int kInStride = 12;  // floats of one slot's network inputs in LDS (<= 10 used)
+constexpr size_t kMaxTraceLds = 160 * 1024;  // LDS of one CU
 enum : int { kWantMarch = 0, kWantVis = 1, kWantThr = 2 };
 
 __device__ __forceinline__ uint32_t slot_mask(uint64_t ballot) { return (uint32_t)(ballot & 0xffffu); }
 
-template <int W, int F, int MAXD>
+// LB: the level's occupancy words are copied into LDS after the weights and
+// the input rows (levels up to about 80^3), so the march's dependent bit
+// reads cost an LDS round trip instead of an L2 one.
+template <int W, int F, int MAXD, bool LB>
 __global__ __launch_bounds__(kTraceBlock) void trace_persistent_kernel(TracerArgs a, NetArgs vis, NetArgs thr,
                                                                        const half_t* __restrict__ packed_vis,
                                                                        const half_t* __restrict__ packed_thr,
@@ -599,6 +605,13 @@ __global__ __launch_bounds__(kTraceBlock) void trace_persistent_kernel(TracerArg
         reinterpret_cast<float4*>(sw + vis.lds_offset)[k] = reinterpret_cast<const float4*>(packed_vis)[k];
     for (uint32_t k = threadIdx.x; k < thr.packed_halves / 8; k += kTraceBlock)
         reinterpret_cast<float4*>(sw + thr.lds_offset)[k] = reinterpret_cast<const float4*>(packed_thr)[k];
+    const uint32_t* bits = a.bits;
+    if constexpr (LB) {
+        uint32_t* s_bits = reinterpret_cast<uint32_t*>(s_in + kTraceWaves * 16 * kInStride);
+        const uint32_t words = (a.R * a.R * a.R + 31u) / 32u;
+        for (uint32_t k = threadIdx.x; k < words; k += kTraceBlock) s_bits[k] = a.bits[k];
+        bits = s_bits;
+    }
     __syncthreads();  // the last block-wide barrier: from here on waves run independently
 
     const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
@@ -645,7 +658,7 @@ __global__ __launch_bounds__(kTraceBlock) void trace_persistent_kernel(TracerArg
         ++n_rounds;
 
         if (live && want == kWantMarch) {
-            if (!march(a, r)) {
+            if (!march(a, bits, r)) {
                 r.state = kEscaped;
                 finish(a, r, b, illum, inferences, bounces, g == 0);
                 live = false;
@@ -749,7 +762,7 @@ struct Tracer {
                  "the throughput network must take 9 inputs (voxel, wo, wi) or 10 (with the LoD level)");
         if (t.L.net.n_output < 3) fail(NLOD_ERR_INVALID_ARGUMENT, "the throughput network needs 3 outputs");
         const size_t in_bytes = (size_t)kTraceWaves * 16 * kInStride * sizeof(float);
-        persistent_ok = fused_pair_ok(v, t, 160 * 1024 - in_bytes);
+        persistent_ok = fused_pair_ok(v, t, kMaxTraceLds - in_bytes);
         if (persistent_ok) {
             uint32_t Fv, Ft, dv, dt;
             layout_features(v.L, Fv, dv);
@@ -839,13 +852,18 @@ struct Tracer {
         const NetArgs at = net_args_of(thr, av.packed_halves);
         hipLaunchKernelGGL(reset_queue_kernel, dim3(1), dim3(64), 0, s, queue.as<uint32_t>());
         NLOD_LAUNCH_CHECK();
+        // the level's occupancy words join the weights in LDS when they fit
+        const size_t bits_bytes = sizeof(uint32_t) * (((size_t)a.R * a.R * a.R + 31) / 32);
+        const bool lds_bits = smem + bits_bytes <= kMaxTraceLds;
+        const size_t launch_smem = lds_bits ? smem + bits_bytes : smem;
         dispatch_wf(vis.L.net.width, F, maxdeg, [&](auto Wc, auto Fc, auto Dc) {
-            auto kern = trace_persistent_kernel<decltype(Wc)::value, decltype(Fc)::value, decltype(Dc)::value>;
-            const int per_cu = std::max(1, occupancy_blocks((const void*)kern, kTraceBlock, smem));
+            constexpr int Wv = decltype(Wc)::value, Fv = decltype(Fc)::value, Dv = decltype(Dc)::value;
+            auto kern = lds_bits ? trace_persistent_kernel<Wv, Fv, Dv, true> : trace_persistent_kernel<Wv, Fv, Dv, false>;
+            const int per_cu = std::max(1, occupancy_blocks((const void*)kern, kTraceBlock, launch_smem));
             const uint32_t waves_needed = div_round_up(a.n_rays, 16);
             const uint32_t blocks = std::max<uint32_t>(
                 1, std::min<uint32_t>(div_round_up(waves_needed, kTraceWaves), (uint32_t)(cu_count() * per_cu)));
-            hipLaunchKernelGGL(kern, dim3(blocks), dim3(kTraceBlock), smem, s, a, av, at,
+            hipLaunchKernelGGL(kern, dim3(blocks), dim3(kTraceBlock), launch_smem, s, a, av, at,
                                static_cast<const half_t*>(vis.packed_fwd(true)),
                                static_cast<const half_t*>(thr.packed_fwd(true)), queue.as<uint32_t>(), illum,
                                inferences, bounces);

@@ -124,6 +124,22 @@ def test_persistent_equals_reference_schedule(scene, name):
         assert 16 * st["throughput_tiles"] >= st["throughput_queries"] >= st["throughput_tiles"]
 
 
+def test_persistent_level_bits_in_global_memory(cuda):
+    """A 128^3 level (256 KB of occupancy bits) does not fit in LDS beside the
+    weights, so the persistent march reads the bits from global memory; it
+    must still equal the reference schedule bit for bit."""
+    tris = synthetic.blob_mesh(30_000, seed=11)
+    table, info = V.voxelize(torch.from_numpy(tris).cuda(), 128)
+    grid = LodGrid(table, info, 2)
+    (vis, _, _), (thr, _, _) = _models(1, seed=13)
+    tr = TR.Tracer(grid, vis, thr)
+    cam = _camera(info, 48, 40)
+    kw = {"lod": 0, "max_depth": 2}
+    got = _render(tr, cam, TR.default_params(schedule=TR.PERSISTENT, **kw), 2)
+    _assert_same(got, _render(tr, cam, TR.default_params(schedule=TR.REFERENCE, **kw), 2), "128^3 level")
+    assert (got[1] >= 1).any()
+
+
 def test_options_equal_oracle(scene):
     """The options that draw extra random numbers or change the arithmetic,
     against the CPU loop (FP16 networks, reference schedule)."""
