@@ -15,15 +15,17 @@
 //               buffers, compaction with atomic counters and a host readback
 //               of each count, the networks through nlod_model_inference.
 //               Any precision; the plain FP32 form is the baseline.
-//   PERSISTENT  FP16 models: one launch for the whole frame.  A wave owns 16
-//               ray slots (one MFMA query tile); each slot runs its ray's state
-//               machine (march -> visibility -> ... -> throughput -> bounce)
-//               and, when the path ends, takes the next ray from a device
-//               queue.  Each round the wave marches its slots, then runs
-//               whichever network more of its slots wait for, with both
-//               networks' weights in LDS.  No host readback, no grid-wide
-//               barrier; a wave exits when the queue is empty and its slots
-//               are done, so the launch is capturable in a hipGraph.
+//   PERSISTENT  FP16 models: one launch for the whole frame.  Each lane of a
+//               wave owns one ray and runs its state machine (march ->
+//               visibility -> ... -> throughput -> bounce); when the path
+//               ends, the lane takes the next ray from a device queue.  Each
+//               round the wave marches its rays, then the lanes waiting for
+//               whichever network more of them wait for write their inputs
+//               to LDS rows, and the whole wave evaluates the rows as 16-query
+//               MFMA tiles (both networks' weights in LDS) and hands each
+//               lane its outputs back through LDS.  No host readback, no
+//               grid-wide barrier; a wave exits when the queue is empty and
+//               its rays are done, so the launch is capturable in a hipGraph.
 // Per ray everything depends only on the ray's own state (the networks give a
 // query's output independently of the other queries of its batch), so the
 // order in which rays are processed cannot change a result.
@@ -581,11 +583,16 @@ __global__ __launch_bounds__(kStageThreads) void bounce_compact_kernel(TracerArg
 // ------------------------------------------------------ persistent schedule
 constexpr int kTraceBlock = 512;
 constexpr int kTraceWaves = kTraceBlock / 64;
+constexpr int kSlots = 64;     // rays in flight per wave: one per lane
 constexpr int kInStride = 12;  // floats of one slot's network inputs in LDS (<= 10 used)
+constexpr int kOutStride = 4;  // floats of one slot's network outputs in LDS (<= 3 used)
 constexpr size_t kMaxTraceLds = 160 * 1024;  // LDS of one CU
 enum : int { kWantMarch = 0, kWantVis = 1, kWantThr = 2 };
 
-__device__ __forceinline__ uint32_t slot_mask(uint64_t ballot) { return (uint32_t)(ballot & 0xffffu); }
+// number of set bits of mask below this lane
+__device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
 
 // LB: the level's occupancy words are copied into LDS after the weights and
 // the input rows (levels up to about 80^3), so the march's dependent bit
@@ -601,49 +608,50 @@ __global__ __launch_bounds__(kTraceBlock) void trace_persistent_kernel(TracerArg
     extern __shared__ float4 smem4[];
     half_t* sw = reinterpret_cast<half_t*>(smem4);
     float* s_in = reinterpret_cast<float*>(sw + vis.packed_halves + thr.packed_halves);
+    float* s_out = s_in + kTraceWaves * kSlots * kInStride;
     for (uint32_t k = threadIdx.x; k < vis.packed_halves / 8; k += kTraceBlock)
         reinterpret_cast<float4*>(sw + vis.lds_offset)[k] = reinterpret_cast<const float4*>(packed_vis)[k];
     for (uint32_t k = threadIdx.x; k < thr.packed_halves / 8; k += kTraceBlock)
         reinterpret_cast<float4*>(sw + thr.lds_offset)[k] = reinterpret_cast<const float4*>(packed_thr)[k];
     const uint32_t* bits = a.bits;
     if constexpr (LB) {
-        uint32_t* s_bits = reinterpret_cast<uint32_t*>(s_in + kTraceWaves * 16 * kInStride);
+        uint32_t* s_bits = reinterpret_cast<uint32_t*>(s_out + kTraceWaves * kSlots * kOutStride);
         const uint32_t words = (a.R * a.R * a.R + 31u) / 32u;
         for (uint32_t k = threadIdx.x; k < words; k += kTraceBlock) s_bits[k] = a.bits[k];
         bits = s_bits;
     }
     __syncthreads();  // the last block-wide barrier: from here on waves run independently
 
-    const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
-    // every lane of slot n keeps the same copy of its ray and reads its
-    // network inputs from the slot's LDS row, which all four write alike
-    float* x = s_in + ((threadIdx.x >> 6) * 16 + n) * kInStride;
+    const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15, wave = threadIdx.x >> 6;
+    // the wave's query rows (network inputs) and output rows, one per slot
+    float* rows = s_in + (size_t)wave * kSlots * kInStride;
+    float* outs = s_out + (size_t)wave * kSlots * kOutStride;
     TRay r;
     bool live = false;
     int want = kWantMarch;
-    uint32_t k = 0, b = 0;        // visibility round and bounce of the slot's ray
+    uint32_t k = 0, b = 0;        // visibility round and bounce of the lane's ray
     uint32_t next = 0, end = 0;   // rays taken from the queue and not yet started
     bool drained = false;
-    // wave-uniform tallies for the frame's statistics: network evaluations of
-    // the tile and the slots they served, per network, and the wave's rounds
+    // wave-uniform tallies for the frame's statistics: network evaluations
+    // (16-query tiles) and the queries they served, per network, and rounds
     uint32_t n_vis_tiles = 0, n_vis_q = 0, n_thr_tiles = 0, n_thr_q = 0, n_rounds = 0;
     for (;;) {
-        // refill free slots in order from the wave's block of the queue
-        uint32_t free_slots = slot_mask(__ballot(!live));
-        while (free_slots && !drained) {
+        // refill free lanes in order from the wave's block of the queue
+        uint64_t free_lanes = __ballot(!live);
+        while (free_lanes && !drained) {
             if (next == end) {
                 uint32_t start = 0;
-                if (lane == 0) start = atomicAdd(queue, 16u);
+                if (lane == 0) start = atomicAdd(queue, (uint32_t)kSlots);
                 start = __shfl(start, 0, 64);
                 if (start >= a.n_rays) {
                     drained = true;
                     break;
                 }
                 next = start;
-                end = min(start + 16u, a.n_rays);
+                end = min(start + (uint32_t)kSlots, a.n_rays);
             }
-            const uint32_t take = min((uint32_t)__popc(free_slots), end - next);
-            const uint32_t rank = (uint32_t)__popc(free_slots & ((1u << n) - 1u));
+            const uint32_t take = min((uint32_t)__popcll(free_lanes), end - next);
+            const uint32_t rank = lane_rank(free_lanes);
             if (!live && rank < take) {
                 raygen(a, next + rank, r);
                 live = true;
@@ -652,75 +660,101 @@ __global__ __launch_bounds__(kTraceBlock) void trace_persistent_kernel(TracerArg
                 b = 0;
             }
             next += take;
-            free_slots = slot_mask(__ballot(!live));
+            free_lanes = __ballot(!live);
         }
-        if (slot_mask(__ballot(live)) == 0) break;  // the queue is empty and every slot is done
+        if (__ballot(live) == 0) break;  // the queue is empty and every lane is done
         ++n_rounds;
 
         if (live && want == kWantMarch) {
             if (!march(a, bits, r)) {
                 r.state = kEscaped;
-                finish(a, r, b, illum, inferences, bounces, g == 0);
+                finish(a, r, b, illum, inferences, bounces, true);
                 live = false;
             } else if (k == a.max_vis - 1) {  // last round of the bounce: the path ends unresolved
-                finish(a, r, b, illum, inferences, bounces, g == 0);
+                finish(a, r, b, illum, inferences, bounces, true);
                 live = false;
             } else {
                 want = kWantVis;
             }
         }
-        const uint32_t n_vis = (uint32_t)__popc(slot_mask(__ballot(live && want == kWantVis)));
-        const uint32_t n_thr = (uint32_t)__popc(slot_mask(__ballot(live && want == kWantThr)));
+        const uint32_t n_vis = (uint32_t)__popcll(__ballot(live && want == kWantVis));
+        const uint32_t n_thr = (uint32_t)__popcll(__ballot(live && want == kWantThr));
         if (n_vis == 0 && n_thr == 0) continue;
-        if (n_thr >= n_vis) {
-            ++n_thr_tiles;
-            n_thr_q += n_thr;
+        // every lane waiting for the network that more lanes wait for writes
+        // its inputs to the row of its rank; the wave then evaluates the rows
+        // 16 at a time (lanes of query n of a tile read row 16 t + n; rows past
+        // the count read row 0, whose output is not used)
+        const bool use_thr = n_thr >= n_vis;
+        const bool mine = live && want == (use_thr ? kWantThr : kWantVis);
+        const uint64_t chosen = __ballot(mine);
+        const uint32_t cnt = (uint32_t)__popcll(chosen);
+        const uint32_t my_row = lane_rank(chosen);
+        if (mine) {
             float in[10];
-            const bool mine = live && want == kWantThr;
-            if (mine) throughput_inputs(a, r, in);
+            if (use_thr) throughput_inputs(a, r, in);
+            else visibility_inputs(a, r, in);
+            const uint32_t n_in = use_thr ? thr.n_in : 6u;
 #pragma unroll
-            for (int j = 0; j < 10; ++j) x[j] = mine ? in[j] : 0.0f;
-            const floatx4 o = run_net<W, F, MAXD>(thr, sw, x);
+            for (int j = 0; j < 10; ++j)
+                if ((uint32_t)j < n_in) rows[my_row * kInStride + j] = in[j];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t tiles = (cnt + 15) / 16;
+        for (uint32_t t = 0; t < tiles; ++t) {
+            const uint32_t q = 16 * t + (uint32_t)n;
+            const float* x = rows + (q < cnt ? q : 0u) * kInStride;
+            if (use_thr) {
+                const floatx4 o = run_net<W, F, MAXD>(thr, sw, x);
+                if (g == 0 && q < cnt) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) outs[q * kOutStride + c] = act_fwd(thr.out_act, o[c]);
+                }
+            } else {
+                const floatx4 o = run_net<W, F, MAXD>(vis, sw, x);
+                if (g == 0 && q < cnt) outs[q * kOutStride] = act_fwd(vis.out_act, o[0]);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (use_thr) {
+            n_thr_tiles += tiles;
+            n_thr_q += cnt;
+        } else {
+            n_vis_tiles += tiles;
+            n_vis_q += cnt;
+        }
+        if (!mine) continue;
+        if (use_thr) {
             float out[3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) out[c] = __shfl(act_fwd(thr.out_act, o[c]), n, 64);  // group 0 holds 0..3
-            if (mine) {
-                bounce(a, r, out, b);
-                if (r.state == kActive) {
-                    ++b;
-                    k = 0;
-                    want = kWantMarch;
-                } else {
-                    finish(a, r, b, illum, inferences, bounces, g == 0);
-                    live = false;
-                }
+            for (int c = 0; c < 3; ++c) out[c] = outs[my_row * kOutStride + c];
+            bounce(a, r, out, b);
+            if (r.state == kActive) {
+                ++b;
+                k = 0;
+                want = kWantMarch;
+            } else {
+                finish(a, r, b, illum, inferences, bounces, true);
+                live = false;
             }
         } else {
-            ++n_vis_tiles;
-            n_vis_q += n_vis;
-            float in[6];
-            const bool mine = live && want == kWantVis;
-            if (mine) visibility_inputs(a, r, in);
-#pragma unroll
-            for (int j = 0; j < 6; ++j) x[j] = mine ? in[j] : 0.0f;
-            const floatx4 o = run_net<W, F, MAXD>(vis, sw, x);
-            const float v = __shfl(act_fwd(vis.out_act, o[0]), n, 64);
-            if (mine) {
-                check_visibility(a, r, v, k);
-                if (r.state == kHit) {
-                    if (b == a.max_depth - 1) {  // the last bounce keeps no intersection
-                        finish(a, r, b, illum, inferences, bounces, g == 0);
-                        live = false;
-                    } else {
-                        want = kWantThr;
-                    }
-                } else if (r.state == kInactive) {
-                    finish(a, r, b, illum, inferences, bounces, g == 0);
+            check_visibility(a, r, outs[my_row * kOutStride], k);
+            if (r.state == kHit) {
+                if (b == a.max_depth - 1) {  // the last bounce keeps no intersection
+                    finish(a, r, b, illum, inferences, bounces, true);
                     live = false;
                 } else {
-                    ++k;
-                    want = kWantMarch;
+                    want = kWantThr;
                 }
+            } else if (r.state == kInactive) {
+                finish(a, r, b, illum, inferences, bounces, true);
+                live = false;
+            } else {
+                ++k;
+                want = kWantMarch;
             }
         }
     }
@@ -761,7 +795,7 @@ struct Tracer {
             fail(NLOD_ERR_INVALID_ARGUMENT,
                  "the throughput network must take 9 inputs (voxel, wo, wi) or 10 (with the LoD level)");
         if (t.L.net.n_output < 3) fail(NLOD_ERR_INVALID_ARGUMENT, "the throughput network needs 3 outputs");
-        const size_t in_bytes = (size_t)kTraceWaves * 16 * kInStride * sizeof(float);
+        const size_t in_bytes = (size_t)kTraceWaves * kSlots * (kInStride + kOutStride) * sizeof(float);
         persistent_ok = fused_pair_ok(v, t, kMaxTraceLds - in_bytes);
         if (persistent_ok) {
             uint32_t Fv, Ft, dv, dt;
