@@ -157,13 +157,13 @@ __device__ __forceinline__ void grid_level(const GridLevels& lv, uint32_t level,
     grid_level_at<F, TT>(lv, level, x0, x1, x2, table + (size_t)lv.offset[level] * F, v);
 }
 
-// Entry e (0 <= e < 8 / F) of a 16-byte block of FP16 entries with F features.
-template <int F>
-__device__ __forceinline__ void entry_of_block(const uint4& blk, uint32_t e, float* out) {
-    const uint32_t w[4] = {blk.x, blk.y, blk.z, blk.w};
-    if constexpr (F == 8) {
+// Entry e (0 <= e < 2 BW / F) of a block of BW 32-bit words of FP16 entries
+// with F features.
+template <int F, int BW>
+__device__ __forceinline__ void entry_of_block(const uint32_t (&w)[BW], uint32_t e, float* out) {
+    if constexpr (2 * BW == F) {  // one entry per block
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < BW; ++k) {
             out[2 * k] = (float)__builtin_bit_cast(half_t, (uint16_t)(w[k] & 0xffffu));
             out[2 * k + 1] = (float)__builtin_bit_cast(half_t, (uint16_t)(w[k] >> 16));
         }
@@ -174,7 +174,10 @@ __device__ __forceinline__ void entry_of_block(const uint4& blk, uint32_t e, flo
 #pragma unroll
         for (uint32_t k = 0; k < WORDS; ++k) {
             const uint32_t wi = word0 + k;
-            sel[k] = wi == 0 ? w[0] : wi == 1 ? w[1] : wi == 2 ? w[2] : w[3];
+            uint32_t x = w[0];
+#pragma unroll
+            for (int j = 1; j < BW; ++j) x = wi == (uint32_t)j ? w[j] : x;
+            sel[k] = x;
         }
         if constexpr (F == 1) {
             const uint32_t bits = (e & 1) ? (sel[0] >> 16) : (sel[0] & 0xffffu);
@@ -189,18 +192,38 @@ __device__ __forceinline__ void entry_of_block(const uint4& blk, uint32_t e, flo
     }
 }
 
+// Loads BW 32-bit words (BW = 2 or 4) from a BW-word-aligned address.
+template <int BW>
+__device__ __forceinline__ void load_block(const half_t* __restrict__ p, uint32_t (&w)[BW]) {
+    if constexpr (BW == 4) {
+        const uint4 t = *reinterpret_cast<const uint4*>(p);
+        w[0] = t.x;
+        w[1] = t.y;
+        w[2] = t.z;
+        w[3] = t.w;
+    } else {
+        const uint2 t = *reinterpret_cast<const uint2*>(p);
+        w[0] = t.x;
+        w[1] = t.y;
+    }
+}
+
 // One level of a grid for one query from an FP16 table, F features in FP32.
-// The two corners of an x-edge usually share a 16-byte block of the table
+// The two corners of an x-edge usually share an aligned block of the table
 // (dense levels: adjacent entries; coherent prime hash: x enters with prime 1,
-// so the entries differ only in the low bits x0 ^ (x0 + 1)): one 16-byte load
-// then serves both, and only the other cases issue a second load.  Same
-// interpolation arithmetic and order as grid_level.
+// so the entries differ only in the low bits x0 ^ (x0 + 1)): one block load
+// then serves both, and only the other cases issue a second load.  Blocks are
+// 8 bytes (F <= 4) or 16 bytes (F = 8): an 8-byte block holds both corners of
+// an edge half the time for F = 2, and takes half the registers of a 16-byte
+// one, which matters to the kernels that keep a wave's gathers of 4 levels in
+// flight at once.  Same interpolation arithmetic and order as grid_level.
 template <int F>
 __device__ __forceinline__ void grid_level_h16(const GridLevels& lv, uint32_t level, float x0, float x1, float x2,
                                                const half_t* __restrict__ table, float* v) {
     // no FMA contraction: the encoding rounds identically in every kernel it is inlined into
 #pragma clang fp contract(off)
-    constexpr uint32_t PER_BLOCK = 8 / F;  // entries per 16-byte block
+    constexpr int BW = F == 8 ? 4 : 2;                // 32-bit words per block
+    constexpr uint32_t PER_BLOCK = 2 * BW / F;        // entries per block
     const float scale = lv.scale[level];
     const uint32_t res = lv.res[level];
     const uint32_t off = lv.offset[level];
@@ -228,27 +251,29 @@ __device__ __forceinline__ void grid_level_h16(const GridLevels& lv, uint32_t le
         i0[p] = grid_index(lv.grid_type, hs, res, cell[0], y, z);
         i1[p] = grid_index(lv.grid_type, hs, res, cell[0] + 1, y, z);
     }
-    uint4 blk0[4], blk1[4];
+    uint32_t blk0[4][BW], blk1[4][BW];
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
-        blk0[p] = *reinterpret_cast<const uint4*>(tab + (size_t)(i0[p] / PER_BLOCK) * PER_BLOCK * F);
+    for (int p = 0; p < 4; ++p) load_block<BW>(tab + (size_t)(i0[p] / PER_BLOCK) * PER_BLOCK * F, blk0[p]);
     bool shared[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
-        // the x0 + 1 corner shares x0's block 3 times in 4 (F = 2); otherwise it needs
-        // its own block.  The load is issued either way, without a branch (a branch
-        // makes the compiler wait for the loads before it): lanes whose corner is
-        // shared read the level's first block, one line that the whole wave shares
+        // the x0 + 1 corner may share x0's block; otherwise it needs its own.
+        // The load is issued either way, without a branch (a branch makes the
+        // compiler wait for the loads before it): lanes whose corner is shared
+        // read the level's first block, one line that the whole wave shares
         // and that stays in the vector L1, so it costs no L2 request per lane.
         shared[p] = i0[p] / PER_BLOCK == i1[p] / PER_BLOCK;
         const size_t b1 = shared[p] ? 0 : (size_t)(i1[p] / PER_BLOCK) * PER_BLOCK * F;
-        blk1[p] = *reinterpret_cast<const uint4*>(tab + b1);
+        load_block<BW>(tab + b1, blk1[p]);
     }
     float vals[8][F];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
-        entry_of_block<F>(blk0[p], i0[p] % PER_BLOCK, vals[2 * p]);
-        entry_of_block<F>(shared[p] ? blk0[p] : blk1[p], i1[p] % PER_BLOCK, vals[2 * p + 1]);
+        entry_of_block<F, BW>(blk0[p], i0[p] % PER_BLOCK, vals[2 * p]);
+        uint32_t b1w[BW];
+#pragma unroll
+        for (int j = 0; j < BW; ++j) b1w[j] = shared[p] ? blk0[p][j] : blk1[p][j];
+        entry_of_block<F, BW>(b1w, i1[p] % PER_BLOCK, vals[2 * p + 1]);
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) {

@@ -581,11 +581,11 @@ __global__ __launch_bounds__(kStageThreads) void bounce_compact_kernel(TracerArg
 }
 
 // ------------------------------------------------------ persistent schedule
-constexpr int kTraceBlock = 512;
+constexpr int kTraceBlock = 768;
 constexpr int kTraceWaves = kTraceBlock / 64;
 constexpr int kSlots = 64;     // rays in flight per wave: one per lane
-constexpr int kInStride = 12;  // floats of one slot's network inputs in LDS (<= 10 used)
-constexpr int kOutStride = 4;  // floats of one slot's network outputs in LDS (<= 3 used)
+constexpr int kInStride = 10;  // floats of one slot's network inputs in LDS (<= 10 used)
+constexpr int kOutStride = 3;  // floats of one slot's network outputs in LDS (<= 3 used)
 constexpr size_t kMaxTraceLds = 160 * 1024;  // LDS of one CU
 enum : int { kWantMarch = 0, kWantVis = 1, kWantThr = 2 };
 
@@ -598,7 +598,7 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask) {
 // the input rows (levels up to about 80^3), so the march's dependent bit
 // reads cost an LDS round trip instead of an L2 one.
 template <int W, int F, int MAXD, bool LB>
-__global__ __launch_bounds__(kTraceBlock) void trace_persistent_kernel(TracerArgs a, NetArgs vis, NetArgs thr,
+__global__ __launch_bounds__(kTraceBlock, 3) void trace_persistent_kernel(TracerArgs a, NetArgs vis, NetArgs thr,
                                                                        const half_t* __restrict__ packed_vis,
                                                                        const half_t* __restrict__ packed_thr,
                                                                        uint32_t* __restrict__ queue,
