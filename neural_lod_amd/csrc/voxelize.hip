@@ -19,7 +19,8 @@
 // (full x extent) small enough for LDS, where the bits are set by LDS atomics:
 //   1. count    per (block of triangles, tile): list entries, from the same
 //               clamped row range the per-triangle routine walks; the tile
-//               range of each triangle is kept for step 3
+//               range of each triangle is kept for step 3.  The same launch
+//               clears the table.
 //   2. scan     tile-major exclusive scans -> list slots and chunk numbers
 //   3. scatter  triangle indices into their tiles' lists
 //   4. tiles    one workgroup per chunk of a tile's list runs the reference's
@@ -702,13 +703,24 @@ __global__ __launch_bounds__(kVoxThreads) void vox_surface_kernel(VoxParams p, c
 // counts[tile][block] = list entries of this block's triangles in the tile;
 // routes[i] = triangle i's packed tile range or marker.
 // LDS: n_tiles counters, then the triangle staging buffer.
+// It also clears the table (zero_words > 0: a grid-stride share per block,
+// as fill_zero_kernel would).
 template <bool SOLID>
 __global__ __launch_bounds__(kVoxThreads) void tile_count_kernel(VoxParams p, const float* __restrict__ tris,
                                                                  TileGeo geo, uint32_t n_blocks,
                                                                  uint32_t* __restrict__ counts,
-                                                                 uint64_t* __restrict__ routes) {
+                                                                 uint64_t* __restrict__ routes,
+                                                                 uint32_t* __restrict__ table, uint64_t zero_words,
+                                                                 int aligned16) {
     extern __shared__ uint32_t cnt[];
     float* s_tri = reinterpret_cast<float*>(cnt + geo.n_tiles);
+    if (zero_words) {
+        const uint64_t i = (uint64_t)blockIdx.x * kVoxThreads + threadIdx.x, stride = (uint64_t)gridDim.x * kVoxThreads;
+        const uint64_t n4 = aligned16 ? zero_words / 4 : 0;
+        uint4* t4 = reinterpret_cast<uint4*>(table);
+        for (uint64_t k = i; k < n4; k += stride) t4[k] = make_uint4(0, 0, 0, 0);
+        for (uint64_t k = n4 * 4 + i; k < zero_words; k += stride) table[k] = 0;
+    }
     for (uint32_t t = threadIdx.x; t < geo.n_tiles; t += kVoxThreads) cnt[t] = 0;
     const uint64_t b0 = (uint64_t)blockIdx.x * geo.tris_per_block;
     const uint64_t b1 = min<uint64_t>(p.n_tri, b0 + geo.tris_per_block);
@@ -1180,8 +1192,9 @@ void launch_voxelize(hipStream_t stream, const VoxParams& p, const TileGeo& geo,
     const size_t scatter_smem = (size_t)geo.n_tiles * 4;
     (void)occupancy_blocks(reinterpret_cast<const void*>(tile_count_kernel<SOLID>), kVoxThreads, count_smem);
     (void)occupancy_blocks(reinterpret_cast<const void*>(tile_scatter_kernel), kVoxThreads, scatter_smem);
+    const uint64_t zero_words = geo.accumulate ? 0 : ((uint64_t)p.G * p.G * p.G + 31) / 32;
     hipLaunchKernelGGL(tile_count_kernel<SOLID>, dim3(n_blocks), dim3(kVoxThreads), count_smem, stream, p, d_tris,
-                       geo, n_blocks, counts, routes);
+                       geo, n_blocks, counts, routes, d_table, zero_words, (int)(((uintptr_t)d_table & 15u) == 0));
     NLOD_LAUNCH_CHECK();
     hipLaunchKernelGGL(tile_scan_rows_kernel, dim3(geo.n_tiles), dim3(kVoxThreads), 0, stream, n_blocks, counts,
                        totals);
@@ -1310,7 +1323,7 @@ void voxelize(hipStream_t stream, const nlod_voxinfo* info, const float* d_tris,
                      " bytes it holds (pass NLOD_VOX_DIRECT, which needs none, or use nlod_voxelize)");
         fixed = noalloc_scratch();
     }
-    if (!(flags & NLOD_VOX_ACCUMULATE)) {
+    if (!(flags & NLOD_VOX_ACCUMULATE) && !tiled) {  // the tiled path's count pass clears the table itself
         const uint32_t blocks = (uint32_t)std::min<uint64_t>(div_round_up(words / 4 + 1, 256), 8192);
         hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, stream, d_table, words,
                            (int)(((uintptr_t)d_table & 15u) == 0));
