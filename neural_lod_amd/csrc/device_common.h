@@ -56,6 +56,44 @@ __device__ __forceinline__ uint32_t grid_index(int grid_type, uint32_t hs, uint3
     return index < hs ? index : index % hs;
 }
 
+// grid_index for the corners of one cell, with the per-level part done once:
+// the stride loop's outcome (which strides it added) and the choice of the
+// hash depend only on the level, and (y + 1) * m = y * m + m in uint32
+// arithmetic, so a cell's rows cost adds instead of multiplies.  The indices
+// equal grid_index's for every input.
+struct LevelIndexer {
+    uint32_t hs, my, mz;  // hashed: the primes; dense: the strides the loop added (0 if not)
+    bool hash;
+};
+
+__device__ __forceinline__ LevelIndexer level_indexer(int grid_type, uint32_t hs, uint32_t res) {
+    uint32_t s1 = 0, s2 = 0, stride = res;
+    if (stride <= hs) {
+        s1 = stride;
+        stride *= res;
+        if (stride <= hs) {
+            s2 = stride;
+            stride *= res;
+        }
+    }
+    LevelIndexer li;
+    li.hs = hs;
+    li.hash = grid_type == GRID_HASH && hs < stride;
+    li.my = li.hash ? 2654435761u : s1;
+    li.mz = li.hash ? 805459861u : s2;
+    return li;
+}
+
+// Index of corner x of the row whose y and z terms are ym = y * my, zm = z * mz.
+__device__ __forceinline__ uint32_t level_index(const LevelIndexer& li, uint32_t x, uint32_t ym, uint32_t zm) {
+    uint32_t index = li.hash ? (x ^ ym ^ zm) : (x + ym + zm);
+    if ((li.hs & (li.hs - 1)) == 0) return index & (li.hs - 1);
+    // dense levels wrap only for positions outside the unit cube: a branch,
+    // so in-range queries never run the division
+    if (__builtin_expect(index >= li.hs, 0)) index %= li.hs;
+    return index;
+}
+
 __device__ __forceinline__ float act_fwd(int a, float v) {
     switch (a) {
         case ACT_RELU: return v > 0.0f ? v : 0.0f;

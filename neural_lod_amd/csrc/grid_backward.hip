@@ -115,13 +115,14 @@ __device__ __forceinline__ void edges(const GridBwdRow& r, const float* x, Emit&
         emit(i, i, 0.0f, 1.0f);
         return;
     }
+    const LevelIndexer li = level_indexer(r.grid_type, r.hs, r.res);
+    const uint32_t y0 = cell[1] * li.my, z0 = cell[2] * li.mz;
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
         const uint32_t dy = p & 1, dz = p >> 1;
         const float wyz = (dy ? fr[1] : 1.0f - fr[1]) * (dz ? fr[2] : 1.0f - fr[2]);
-        const uint32_t y = cell[1] + dy, z = cell[2] + dz;
-        emit(grid_index(r.grid_type, r.hs, r.res, cell[0], y, z), grid_index(r.grid_type, r.hs, r.res, cell[0] + 1, y, z),
-             fr[0], wyz);
+        const uint32_t ym = dy ? y0 + li.my : y0, zm = dz ? z0 + li.mz : z0;
+        emit(level_index(li, cell[0], ym, zm), level_index(li, cell[0] + 1, ym, zm), fr[0], wyz);
     }
 }
 

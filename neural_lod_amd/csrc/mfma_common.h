@@ -245,11 +245,15 @@ __device__ __forceinline__ void grid_level_h16(const GridLevels& lv, uint32_t le
     // issued before any is used, so a lane keeps its up to 8 gathers in flight
     // together instead of waiting for each in turn.
     uint32_t i0[4], i1[4];
+    {
+        const LevelIndexer li = level_indexer(lv.grid_type, hs, res);
+        const uint32_t y0 = cell[1] * li.my, z0 = cell[2] * li.mz;
 #pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        const uint32_t y = cell[1] + (p & 1), z = cell[2] + ((p >> 1) & 1);
-        i0[p] = grid_index(lv.grid_type, hs, res, cell[0], y, z);
-        i1[p] = grid_index(lv.grid_type, hs, res, cell[0] + 1, y, z);
+        for (int p = 0; p < 4; ++p) {
+            const uint32_t ym = (p & 1) ? y0 + li.my : y0, zm = ((p >> 1) & 1) ? z0 + li.mz : z0;
+            i0[p] = level_index(li, cell[0], ym, zm);
+            i1[p] = level_index(li, cell[0] + 1, ym, zm);
+        }
     }
     uint32_t blk0[4][BW], blk1[4][BW];
 #pragma unroll
@@ -343,10 +347,14 @@ __device__ __forceinline__ void grid_level_h16_pair(const GridLevels& lv, uint32
         return;
     }
     uint32_t mine[4][WORDS], other[4][WORDS];
+    {
+        const LevelIndexer li = level_indexer(lv.grid_type, hs, res);
+        const uint32_t y0 = cell[1] * li.my, z0 = cell[2] * li.mz;
 #pragma unroll
-    for (int p = 0; p < 4; ++p)
-        load_words(grid_index(lv.grid_type, hs, res, cell[0] + s, cell[1] + (p & 1), cell[2] + ((p >> 1) & 1)),
-                   mine[p]);
+        for (int p = 0; p < 4; ++p)
+            load_words(level_index(li, cell[0] + s, (p & 1) ? y0 + li.my : y0, ((p >> 1) & 1) ? z0 + li.mz : z0),
+                       mine[p]);
+    }
 #pragma unroll
     for (int p = 0; p < 4; ++p)
 #pragma unroll
