@@ -399,10 +399,14 @@ __device__ __forceinline__ void encode_window_f32(const EncProgram& P, const TT*
             const int n_levels = (int)s.lv.n_levels;
             const float x0 = x[0], x1 = x[1], x2 = x[2];
             const TT* table = params + s.param_offset;
+            // rows of level l_begin + t start at row0 + t F; row0 = 0 when the
+            // window starts at or after the segment (always for a segment at a
+            // window boundary), so the placement is by constant slot
+            const int row0 = (int)r0 + l_begin * F - (int)w0;
 #pragma unroll
             for (int t = 0; t < 8 / F; ++t) {
                 const int l = l_begin + t;
-                const int row = (int)r0 + l * F - (int)w0;
+                const int row = row0 + t * F;
                 // gather unconditionally (clamped level: indices stay inside it), place if valid
                 float v[F];
                 if constexpr (std::is_same<TT, half_t>::value)
@@ -410,8 +414,13 @@ __device__ __forceinline__ void encode_window_f32(const EncProgram& P, const TT*
                 else
                     grid_level<F, TT>(s.lv, (uint32_t)min(l, n_levels - 1), x0, x1, x2, table, v);
                 const bool ok = l < n_levels && row >= 0 && row < 8;
+                if (row0 == 0) {
 #pragma unroll
-                for (int f = 0; f < F; ++f) place(win, ok ? row + f : -1, v[f]);
+                    for (int f = 0; f < F; ++f) win[t * F + f] = l < n_levels ? v[f] : win[t * F + f];
+                } else {
+#pragma unroll
+                    for (int f = 0; f < F; ++f) place(win, ok ? row + f : -1, v[f]);
+                }
             }
         } else if (s.type == SEG_SH) {
             const uint32_t sh0 = r0 + s.n_pad;  // SH pad rows (1.0) come first
@@ -431,7 +440,15 @@ __device__ __forceinline__ void encode_window_f32(const EncProgram& P, const TT*
                     Z[k] = Z[k - 1] * vz;
                 }
                 const int base = (int)sh0 - (int)w0;
-                sh_basis<MAXD>(s.degree, X, Y, Z, [&](int k, float v) { place(win, base + k, v); });
+                if ((base & 7) == 0) {
+                    // coefficient k lands in slot k & 7 of window -base / 8: one select each
+                    const int wk = -base >> 3;
+                    sh_basis<MAXD>(s.degree, X, Y, Z, [&](int k, float v) {
+                        if ((k >> 3) == wk) win[k & 7] = v;
+                    });
+                } else {
+                    sh_basis<MAXD>(s.degree, X, Y, Z, [&](int k, float v) { place(win, base + k, v); });
+                }
             }
         } else {
 #pragma unroll
