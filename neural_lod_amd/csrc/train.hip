@@ -149,8 +149,16 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(TrainHyper h, uint32_t
     }
 }
 
+// bias_correction(a, t): the step-size factor of adam.h's update for a
+// parameter at its t-th step.  Parameters that took every step share t with
+// the optimizer, so a kernel computes the factor for its step count T once and
+// passes it as bc_T (the same float expression, hence the same value).
+__device__ __forceinline__ float bias_correction(const AdamParams& a, uint32_t t) {
+    return sqrtf(1.0f - powf(a.beta2, (float)t)) / (1.0f - powf(a.beta1, (float)t));
+}
+
 __device__ __forceinline__ float adam_one(const AdamParams& a, const TrainState& st, bool matrix, float w, float g,
-                                          float& m, float& v, uint32_t& t, bool& skip) {
+                                          float& m, float& v, uint32_t& t, bool& skip, uint32_t T, float bc_T) {
     // adam.h:75-118
     float lr = st.lr;
     if (matrix) {
@@ -165,7 +173,7 @@ __device__ __forceinline__ float adam_one(const AdamParams& a, const TrainState&
     m = a.beta1 * m + (1.0f - a.beta1) * g;
     v = a.beta2 * v + (1.0f - a.beta2) * g2;
     t = t + 1;
-    lr *= sqrtf(1.0f - powf(a.beta2, (float)t)) / (1.0f - powf(a.beta1, (float)t));
+    lr *= t == T ? bc_T : bias_correction(a, t);
     float lo = 0.0f, hi = 3.402823466e38f;
     if (a.adabound) {
         const float k = (1.0f - a.beta2) * (float)st.step;
@@ -189,6 +197,12 @@ __global__ __launch_bounds__(kBlock) void adam_kernel(AdamParams a, uint64_t n, 
     const TrainState st = *stp;
     const float scale = st.applied_scale;
     const uint64_t n4 = (n + 3) / 4;
+    // the bias correction at the optimizer's step count, once per block
+    __shared__ float s_bc;
+    if (threadIdx.x == 0) s_bc = st.step ? bias_correction(a, st.step) : 0.0f;
+    __syncthreads();
+    const uint32_t T = st.step;
+    const float bc_T = s_bc;
     for (uint64_t q = (uint64_t)blockIdx.x * kBlock + threadIdx.x; q < n4; q += (uint64_t)gridDim.x * kBlock) {
         const uint64_t i0 = q * 4;
         if (i0 + 4 <= n) {
@@ -209,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void adam_kernel(AdamParams a, uint64_t n, 
                 bool skip;
                 float mk = me[k], vk = ve[k];
                 uint32_t tk = te[k];
-                const float nw = adam_one(a, st, i0 + k < n_matrix, we[k], ge[k] / scale, mk, vk, tk, skip);
+                const float nw = adam_one(a, st, i0 + k < n_matrix, we[k], ge[k] / scale, mk, vk, tk, skip, T, bc_T);
                 if (!skip) {
                     we[k] = nw;
                     me[k] = mk;
@@ -234,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void adam_kernel(AdamParams a, uint64_t n, 
                 bool skip;
                 float mk = m1[i], vk = m2[i];
                 uint32_t tk = steps[i];
-                const float nw = adam_one(a, st, i < n_matrix, w[i], gi / scale, mk, vk, tk, skip);
+                const float nw = adam_one(a, st, i < n_matrix, w[i], gi / scale, mk, vk, tk, skip, T, bc_T);
                 if (skip) continue;
                 w[i] = nw;
                 m1[i] = mk;
