@@ -23,8 +23,8 @@ class GridBackward {
     void run(hipStream_t s, const float* in, uint32_t N, const float* d_enc, float* grads, bool overwrite);
     // attaches the owner's epoch counter to every buffer (see DevBuf)
     void track(uint64_t* epoch) {
-        for (DevBuf* b : {&d_rows_, &d_bins_, &counts_, &run_addr_, &totals_, &chunk_start_, &meta_, &records_,
-                          &partial_})
+        for (DevBuf* b : {&d_rows_, &d_bins_, &counts_, &run_addr_, &totals_, &chunk_start_, &apply_list_, &meta_,
+                          &records_, &partial_})
             b->epoch = epoch;
     }
 
@@ -42,7 +42,7 @@ class GridBackward {
     DevBuf d_rows_, d_bins_;
     // per (tile, repetition, query block): run length (scanned in place into the
     // run's first record in the tile's order) and the run's first slot
-    DevBuf counts_, run_addr_, totals_, chunk_start_, meta_, records_, partial_;
+    DevBuf counts_, run_addr_, totals_, chunk_start_, apply_list_, meta_, records_, partial_;
 };
 
 }  // namespace nlod

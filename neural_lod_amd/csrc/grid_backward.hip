@@ -165,37 +165,59 @@ __global__ __launch_bounds__(kBlock) void scan_rows_kernel(uint32_t n_qblocks, c
 }
 
 // Pass 2b (one block): group numbering (a group = up to chunk_recs records
-// of one tile).  meta[0] = groups.
+// of one tile).  meta[0] = groups.  Also lists, in order, the tiles the apply
+// pass has work for (several groups, or none when overwriting): meta[2] of them.
 __device__ __forceinline__ uint32_t groups_of(uint32_t total, uint32_t chunk_recs) {
     return (total + chunk_recs - 1) / chunk_recs;
 }
 
-__global__ __launch_bounds__(kBlock) void scan_bins_kernel(uint32_t n_bins, uint32_t chunk_recs,
+__device__ __forceinline__ bool needs_apply(uint32_t groups, int overwrite) {
+    return groups >= 2 || (groups == 0 && overwrite);
+}
+
+__global__ __launch_bounds__(kBlock) void scan_bins_kernel(uint32_t n_bins, uint32_t chunk_recs, int overwrite,
                                                            const uint32_t* __restrict__ totals,
                                                            uint32_t* __restrict__ group_start,
+                                                           uint32_t* __restrict__ apply_list,
                                                            uint32_t* __restrict__ meta) {
-    __shared__ uint32_t part_c[kBlock];
+    // both counts scanned at once: groups in the high word, listed tiles in the
+    // low word (their sum stays below 2^32, so no carry crosses)
+    __shared__ uint64_t s_wave[kBlock / 64];
     const uint32_t per = (n_bins + kBlock - 1) / kBlock;
     const uint32_t b0 = min(n_bins, threadIdx.x * per), b1 = min(n_bins, b0 + per);
-    uint32_t sc = 0;
-    for (uint32_t i = b0; i < b1; ++i) sc += groups_of(totals[i], chunk_recs);
-    part_c[threadIdx.x] = sc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t ac = 0;
-        for (int i = 0; i < kBlock; ++i) {
-            const uint32_t vc = part_c[i];
-            part_c[i] = ac;
-            ac += vc;
-        }
-        meta[0] = ac;
-        group_start[n_bins] = ac;
-    }
-    __syncthreads();
-    uint32_t ac = part_c[threadIdx.x];
+    uint64_t mine = 0;
     for (uint32_t i = b0; i < b1; ++i) {
+        const uint32_t g = groups_of(totals[i], chunk_recs);
+        mine += ((uint64_t)g << 32) | (needs_apply(g, overwrite) ? 1u : 0u);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t x = mine;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    uint64_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const uint64_t v = s_wave[w];
+        if (w < wave) before += v;
+        total += v;
+    }
+    const uint64_t excl = before + x - mine;
+    if (threadIdx.x == 0) {
+        meta[0] = (uint32_t)(total >> 32);
+        meta[2] = (uint32_t)total;
+        group_start[n_bins] = (uint32_t)(total >> 32);
+    }
+    uint32_t ac = (uint32_t)(excl >> 32), aa = (uint32_t)excl;
+    for (uint32_t i = b0; i < b1; ++i) {
+        const uint32_t g = groups_of(totals[i], chunk_recs);
         group_start[i] = ac;
-        ac += groups_of(totals[i], chunk_recs);
+        ac += g;
+        if (needs_apply(g, overwrite)) apply_list[aa++] = i;
     }
 }
 
@@ -472,29 +494,35 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(uint32_t n_bins, uint32_
     }
 }
 
-// Pass 4: tiles with several groups, and (overwrite) tiles with none.  grid (tiles, kTileFloats / kBlock).
+// Pass 4: tiles with several groups, and (overwrite) tiles with none: the
+// meta[2] tiles of apply_list, kTileFloats / kBlock work items each, taken
+// by a persistent grid.
 __global__ __launch_bounds__(kBlock) void apply_kernel(const GridBwdBin* __restrict__ bins,
                                                        const uint32_t* __restrict__ group_start,
                                                        const uint32_t* __restrict__ meta,
+                                                       const uint32_t* __restrict__ apply_list,
                                                        const long long* __restrict__ partial,
                                                        float* __restrict__ grads, uint32_t F, int overwrite) {
-    const uint32_t b = blockIdx.x;
-    const uint32_t c0 = group_start[b], c1 = group_start[b + 1];
-    if (c1 - c0 == 1 || (c1 == c0 && !overwrite)) return;
-    const uint32_t e = blockIdx.y * kBlock + threadIdx.x;
-    const GridBwdBin bin = bins[b];
-    if (e >= bin.n_entries * F) return;
-    float* dst = grads + bin.float_offset + e;
-    if (c1 == c0) {
-        *dst = 0.0f;
-        return;
-    }
-    long long s = 0;
-    for (uint32_t c = c0; c < c1; ++c) s += partial[(size_t)c * kTileFloats + e];
+    constexpr uint32_t kItems = kTileFloats / kBlock;
+    const uint32_t n_items = meta[2] * kItems;
     const uint32_t mbits = meta[1];
-    const float g = finite_bits(mbits) ? ldexpf((float)s, -fixed_shift(mbits)) : __uint_as_float(0x7fc00000u);
-    if (overwrite) *dst = g;
-    else *dst += g;
+    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const uint32_t b = apply_list[item / kItems];
+        const uint32_t c0 = group_start[b], c1 = group_start[b + 1];
+        const uint32_t e = (item % kItems) * kBlock + threadIdx.x;
+        const GridBwdBin bin = bins[b];
+        if (e >= bin.n_entries * F) continue;
+        float* dst = grads + bin.float_offset + e;
+        if (c1 == c0) {
+            *dst = 0.0f;
+            continue;
+        }
+        long long sum = 0;
+        for (uint32_t c = c0; c < c1; ++c) sum += partial[(size_t)c * kTileFloats + e];
+        const float g = finite_bits(mbits) ? ldexpf((float)sum, -fixed_shift(mbits)) : __uint_as_float(0x7fc00000u);
+        if (overwrite) *dst = g;
+        else *dst += g;
+    }
 }
 
 template <typename Fn>
@@ -615,6 +643,7 @@ void GridBackward::prepare(uint32_t N) {
     run_addr_.ensure(sizeof(uint32_t) * (size_t)count_rows_ * nqb);
     totals_.ensure(sizeof(uint32_t) * n_bins_);
     chunk_start_.ensure(sizeof(uint32_t) * (n_bins_ + 1));
+    apply_list_.ensure(sizeof(uint32_t) * n_bins_);
     meta_.ensure(sizeof(uint32_t) * 4);
     const size_t rec_words = F_ + 2 <= 4 ? 4 : F_ + 2 <= 8 ? 8 : 16;
     // every (row, query block) has a region for its largest record count
@@ -656,10 +685,11 @@ void GridBackward::run(hipStream_t s, const float* in, uint32_t N, const float* 
         hipLaunchKernelGGL(scan_rows_kernel, dim3(n_bins_), dim3(kBlock), 0, s, nqb, d_bins_.as<GridBwdBin>(),
                            counts_.as<uint32_t>(), totals_.as<uint32_t>());
         NLOD_LAUNCH_CHECK();
-        hipLaunchKernelGGL(scan_bins_kernel, dim3(1), dim3(kBlock), 0, s, n_bins_, kGroupRecs,
-                           totals_.as<uint32_t>(), chunk_start_.as<uint32_t>(), meta_.as<uint32_t>());
-        NLOD_LAUNCH_CHECK();
         const int ow = (overwrite && q0 == 0) ? 1 : 0;
+        hipLaunchKernelGGL(scan_bins_kernel, dim3(1), dim3(kBlock), 0, s, n_bins_, kGroupRecs, ow,
+                           totals_.as<uint32_t>(), chunk_start_.as<uint32_t>(), apply_list_.as<uint32_t>(),
+                           meta_.as<uint32_t>());
+        NLOD_LAUNCH_CHECK();
         by_F(F_, [&](auto Fc) {
             constexpr int F = decltype(Fc)::value;
             hipLaunchKernelGGL(reduce_kernel<F>, dim3((uint32_t)n_groups), dim3(kBlock), 0, s, n_bins_, nqb,
@@ -668,8 +698,10 @@ void GridBackward::run(hipStream_t s, const float* in, uint32_t N, const float* 
                                records_.as<uint32_t>(), partial_.as<long long>(), grads, ow);
         });
         NLOD_LAUNCH_CHECK();
-        hipLaunchKernelGGL(apply_kernel, dim3(n_bins_, kTileFloats / kBlock), dim3(kBlock), 0, s,
-                           d_bins_.as<GridBwdBin>(), chunk_start_.as<uint32_t>(), meta_.as<uint32_t>(),
+        const uint32_t apply_blocks =
+            std::min<uint32_t>(n_bins_ * (kTileFloats / kBlock), (uint32_t)cu_count() * 8);
+        hipLaunchKernelGGL(apply_kernel, dim3(apply_blocks), dim3(kBlock), 0, s, d_bins_.as<GridBwdBin>(),
+                           chunk_start_.as<uint32_t>(), meta_.as<uint32_t>(), apply_list_.as<uint32_t>(),
                            partial_.as<long long>(), grads, F_, ow);
         NLOD_LAUNCH_CHECK();
     }

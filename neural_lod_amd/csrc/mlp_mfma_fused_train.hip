@@ -387,22 +387,65 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
 }
 
 // out[e] (+)= sum over partial rows, in row order; 8 rows in flight per thread.
+// Sum of the per-block weight-gradient rows.  A block takes 64 consecutive
+// parameters; its 16 row slices (four lanes groups of each wave) each sum a
+// contiguous range of rows, four parameters per lane, with every load of a
+// slice in flight together, and the slices are added in a fixed order, so the
+// result does not depend on timing.
+constexpr int kReduceSlices = 16;
+constexpr int kReduceCols = 64;
+
 __global__ __launch_bounds__(256) void partial_reduce_kernel(uint32_t n, uint32_t rows,
                                                              const float* __restrict__ partial,
                                                              float* __restrict__ out, int accumulate) {
-    const uint32_t e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
-    float s = 0.0f;
-    uint32_t r = 0;
-    for (; r + 8 <= rows; r += 8) {
-        float v[8];
+    __shared__ float4 s_sum[kReduceSlices][kReduceCols / 4];
+    const uint32_t slice = threadIdx.x / 16, col4 = threadIdx.x % 16;
+    const uint32_t e0 = blockIdx.x * kReduceCols + col4 * 4;
+    const uint32_t per = (rows + kReduceSlices - 1) / kReduceSlices;
+    const uint32_t r0 = min(rows, slice * per), r1 = min(rows, r0 + per);
+    float4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    if ((n & 3) == 0 && e0 + 4 <= n) {
+        const float4* p = reinterpret_cast<const float4*>(partial + e0);
+        const size_t stride4 = n / 4;
+        uint32_t r = r0;
+        for (; r + 8 <= r1; r += 8) {
+            float4 v[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = partial[(size_t)(r + k) * n + e];
+            for (int k = 0; k < 8; ++k) v[k] = p[(size_t)(r + k) * stride4];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) s += v[k];
+            for (int k = 0; k < 8; ++k) {
+                acc.x += v[k].x;
+                acc.y += v[k].y;
+                acc.z += v[k].z;
+                acc.w += v[k].w;
+            }
+        }
+        for (; r < r1; ++r) {
+            const float4 v = p[(size_t)r * stride4];
+            acc.x += v.x;
+            acc.y += v.y;
+            acc.z += v.z;
+            acc.w += v.w;
+        }
+    } else {
+        for (uint32_t r = r0; r < r1; ++r) {
+            const float* row = partial + (size_t)r * n;
+            if (e0 + 0 < n) acc.x += row[e0 + 0];
+            if (e0 + 1 < n) acc.y += row[e0 + 1];
+            if (e0 + 2 < n) acc.z += row[e0 + 2];
+            if (e0 + 3 < n) acc.w += row[e0 + 3];
+        }
     }
-    for (; r < rows; ++r) s += partial[(size_t)r * n + e];
-    out[e] = accumulate ? out[e] + s : s;
+    s_sum[slice][col4] = acc;
+    __syncthreads();
+    if (threadIdx.x >= kReduceCols) return;
+    const uint32_t e = blockIdx.x * kReduceCols + threadIdx.x;
+    if (e >= n) return;
+    const float* flat = reinterpret_cast<const float*>(&s_sum[0][0]);
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < kReduceSlices; ++k) sum += flat[k * kReduceCols + threadIdx.x];
+    out[e] = accumulate ? out[e] + sum : sum;
 }
 
 template <typename Fn>
@@ -483,8 +526,8 @@ void mfma_fused_train(hipStream_t stream, const Layout& L, const void* packed, c
 
 void partial_reduce(hipStream_t stream, uint32_t n, uint32_t rows, const float* partial, float* out, bool accumulate) {
     if (n == 0) return;
-    hipLaunchKernelGGL(partial_reduce_kernel, dim3(div_round_up(n, 256)), dim3(256), 0, stream, n, rows, partial, out,
-                       accumulate ? 1 : 0);
+    hipLaunchKernelGGL(partial_reduce_kernel, dim3(div_round_up(n, kReduceCols)), dim3(256), 0, stream, n, rows,
+                       partial, out, accumulate ? 1 : 0);
     NLOD_LAUNCH_CHECK();
 }
 
