@@ -149,6 +149,12 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
 #pragma unroll
     for (int t = 0; t < S::ZT; ++t) acc_in[t] = floatx4{0.f, 0.f, 0.f, 0.f};
     double lsum = 0.0;
+    // single-output networks: only lane group 0 holds an output in a round, so
+    // the loss values (the costly part of the loss) are evaluated every fourth
+    // round with all four groups, one element each
+    const bool single = a.n_output == 1;
+    float pend_p = 0.0f, pend_t = 0.0f;
+    bool pend_ok = false;
 
     // weight-fragment offsets (halves) of the forward and backward packings
     const size_t f_hidden0 = (size_t)MT * kc0 * 512;
@@ -222,20 +228,38 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
 
         // ---- loss and delta_out (rows 4g+i = outputs of query n)
         half8 bo;
+        float p0 = 0.0f;  // output 0 of query n after the output activation (group 0)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const uint32_t oi = 4 * g + i;
             float d = 0.0f;
             if (valid && oi < a.n_output) {
                 const float p = act_fwd(a.out_act, o[i]);
-                double v;
-                float gr;
-                loss_elem(a.loss_type, p, tg[i], n_total, scale, v, gr);
-                lsum += v;
+                const float gr = loss_grad(a.loss_type, p, tg[i], n_total, scale);
+                if (single) {
+                    p0 = p;
+                } else {
+                    lsum += loss_value(a.loss_type, p, tg[i], n_total);
+                }
                 d = gr * act_bwd(a.out_act, p);
             }
             bo[i] = (half_t)d;
             bo[i + 4] = (half_t)0.0f;
+        }
+        if (single) {
+            // lane (g, n) keeps the element of round 4k + g and evaluates the
+            // values of four rounds at once
+            const uint32_t slot = r & 3u;
+            const float sp = __shfl(p0, n, 64), st = __shfl(tg[0], n, 64);
+            if ((uint32_t)g == slot) {
+                pend_p = sp;
+                pend_t = st;
+                pend_ok = valid;
+            }
+            if (slot == 3u || r + 1 == rounds) {
+                if (pend_ok) lsum += loss_value(a.loss_type, pend_p, pend_t, n_total);
+                pend_ok = false;
+            }
         }
 
         load_round(r + 1, xn, tn);
@@ -386,11 +410,10 @@ __global__ __launch_bounds__(kThreads) void fused_train_kernel(FusedArgs a, cons
     }
 }
 
-// out[e] (+)= sum over partial rows, in row order; 8 rows in flight per thread.
-// Sum of the per-block weight-gradient rows.  A block takes 64 consecutive
-// parameters; its 16 row slices (four lanes groups of each wave) each sum a
-// contiguous range of rows, four parameters per lane, with every load of a
-// slice in flight together, and the slices are added in a fixed order, so the
+// out[e] (+)= sum of the per-block weight-gradient rows.  A block takes 64
+// consecutive parameters; its 16 row slices (the four 16-lane groups of each
+// of its four waves) each sum a contiguous range of rows, four parameters per
+// lane and 8 rows in flight, and the slices are added in a fixed order, so the
 // result does not depend on timing.
 constexpr int kReduceSlices = 16;
 constexpr int kReduceCols = 64;
