@@ -37,9 +37,9 @@ BYTES_PER_QUERY = ENC_BYTES_PER_QUERY + 48 * 2 + 4
 FLOP_PER_QUERY = 2 * (48 * 64 + 3 * 64 * 64 + 64 * 1)
 # HBM bytes of one grid backward at 2^18 queries of the visibility network,
 # from FETCH_SIZE and WRITE_SIZE passes calibrated by a 256 MiB copy
-# (bench.py --profile-grid-backward; 346.1 MB fetched + 370.6 MB written: the
+# (bench.py --profile-grid-backward; 345.9 MB fetched + 358.5 MB written: the
 # 16-byte edge records are written by the bin pass and read by the reduce)
-GRID_BWD_TRAFFIC_BYTES = 716.7e6
+GRID_BWD_TRAFFIC_BYTES = 704.4e6
 # HBM bytes of one encoding pass (its two launches) at 2^20 queries, from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes calibrated by a 256 MiB copy
 # (bench.py --profile-encode; 420.5 MB fetched + 150.3 MB written)
