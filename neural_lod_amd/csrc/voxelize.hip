@@ -16,13 +16,16 @@
 // is its bound.
 //
 // Tiled (row-major tables): the grid is cut into tiles of T x T voxel rows
-// (full x extent) small enough for LDS, where the bits are set by LDS atomics:
+// (full x extent) small enough for LDS, where the bits are set by LDS atomics.
+// Four launches:
 //   1. count    per (block of triangles, tile): list entries, from the same
 //               clamped row range the per-triangle routine walks; the tile
 //               range of each triangle is kept for step 3.  The same launch
 //               clears the table.
-//   2. scan     tile-major exclusive scans -> list slots and chunk numbers
-//   3. scatter  triangle indices into their tiles' lists
+//   2. scan     per tile: exclusive scan over the count blocks, and the total
+//   3. scatter  each block scans the tile totals (list starts; block 0 also
+//               numbers the chunks) and writes its triangles' indices into
+//               their tiles' lists
 //   4. tiles    one workgroup per chunk of a tile's list runs the reference's
 //               routine restricted to the tile's rows into an LDS copy of the
 //               tile, then merges its nonzero words into the table (plain
@@ -32,8 +35,10 @@
 //               ranges with a suffix-XOR scan per row (XOR is linear, so the
 //               chunks of a tile still combine by XOR).  Surface mode spreads
 //               the voxel columns of a batch of triangles over the lanes.
-// Triangles touching more than kMaxTiles tiles take the direct path after
-// the tiled pass; Morton-ordered output always does.
+//               Triangles touching more than kMaxTiles tiles then run in the
+//               same launch with global atomics (every merge is an atomic
+//               when there are any).
+// Morton-ordered output always takes the direct path.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
@@ -563,7 +568,7 @@ __device__ Rows solid_triangle(const VoxParams& p, const float* t, Sink* out, co
 constexpr int kMaxTiles = 9;            // triangles touching more tiles take the direct path
 constexpr uint32_t kTileWords = 16384;  // max LDS words of one tile (64 KiB)
 constexpr uint32_t kMaxTileCount = 16384;
-constexpr uint32_t kMinTile = 8;
+constexpr uint32_t kMinTile = 16;
 
 // Tile geometry of a row-major table: tiles of T x T rows (full x extent).
 struct TileGeo {
@@ -611,26 +616,13 @@ __device__ __forceinline__ void stage_triangles(float* s_tri, const float* __res
 }
 
 // Direct path, solid: one triangle per lane, global atomics.
-// routes (nullable): only the triangles routed to the direct path.
 __global__ __launch_bounds__(kVoxThreads) void vox_solid_kernel(VoxParams p, const float* __restrict__ tris,
-                                                                const uint64_t* __restrict__ routes,
                                                                 uint32_t* __restrict__ table) {
     __shared__ float s_tri[kVoxThreads * 9];
     const Rows all{0, (int)p.G - 1, 0, (int)p.G - 1};
     for (uint64_t base = (uint64_t)blockIdx.x * kVoxThreads; base < p.n_tri;
          base += (uint64_t)gridDim.x * kVoxThreads) {
         const uint32_t nt = (uint32_t)min<uint64_t>(kVoxThreads, p.n_tri - base);
-        if (routes) {
-            const uint64_t i = base + threadIdx.x;
-            if (threadIdx.x < nt && routes[i] == kRouteDirect) {
-                float t[9];
-#pragma unroll
-                for (int k = 0; k < 9; ++k) t[k] = tris[i * 9 + k];
-                GlobalSink<true> out{table};
-                solid_triangle<false>(p, t, &out, all);
-            }
-            continue;
-        }
         stage_triangles(s_tri, tris, base, nt);
         __syncthreads();
         if (threadIdx.x < nt) {
@@ -647,10 +639,8 @@ __global__ __launch_bounds__(kVoxThreads) void vox_solid_kernel(VoxParams p, con
 // Direct path, surface: swept along x so that the bits of one row merge into
 // one global atomic per word.  Each lane sets up one triangle of a batch and
 // the batch's x columns are spread over the block (surface_block), so a wave
-// does not wait on its largest triangle.  routes (nullable): only the
-// triangles routed to the direct path, one per lane.
+// does not wait on its largest triangle.
 __global__ __launch_bounds__(kVoxThreads) void vox_surface_kernel(VoxParams p, const float* __restrict__ tris,
-                                                                  const uint64_t* __restrict__ routes,
                                                                   uint32_t* __restrict__ table) {
     __shared__ float s_tri[kVoxThreads * 9];
     __shared__ SurfTri s_st[kVoxThreads];
@@ -659,38 +649,16 @@ __global__ __launch_bounds__(kVoxThreads) void vox_surface_kernel(VoxParams p, c
     for (uint64_t base = (uint64_t)blockIdx.x * kVoxThreads; base < p.n_tri;
          base += (uint64_t)gridDim.x * kVoxThreads) {
         const uint32_t nt = (uint32_t)min<uint64_t>(kVoxThreads, p.n_tri - base);
-        float t[9];
-        bool have = threadIdx.x < nt;
-        if (routes) {
-            have = have && routes[base + threadIdx.x] == kRouteDirect;
-            if (have) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) t[k] = tris[(base + threadIdx.x) * 9 + k];
-            }
-        } else {
-            stage_triangles(s_tri, tris, base, nt);
-            __syncthreads();
-            if (have) {
-#pragma unroll
-                for (int k = 0; k < 9; ++k) t[k] = s_tri[threadIdx.x * 9 + k];
-            }
-            __syncthreads();
-        }
-        if (routes) {
-            // the few triangles the tiled path hands over: one per lane
-            if (!have) continue;
-            const SurfTri st = surface_setup(p, t, all, true);
-            if (st.axis != 0) continue;
-            GlobalSink<false> out{table};
-            const uint32_t cols = surface_columns<0>(st);
-            for (uint32_t j = 0; j < cols; ++j) surface_column<0>(p, st, j, out);
-            out.flush();
-            continue;
-        }
-        // a whole batch: the x columns of its 256 triangles spread over the block
+        stage_triangles(s_tri, tris, base, nt);
+        __syncthreads();
         SurfTri st;
         st.axis = -1;
-        if (have) st = surface_setup(p, t, all, true);
+        if (threadIdx.x < nt) {
+            float t[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) t[k] = s_tri[threadIdx.x * 9 + k];
+            st = surface_setup(p, t, all, true);
+        }
         s_st[threadIdx.x] = st;
         __syncthreads();
         GlobalSink<false> out{table};
@@ -698,10 +666,35 @@ __global__ __launch_bounds__(kVoxThreads) void vox_surface_kernel(VoxParams p, c
     }
 }
 
+// One triangle of the direct route inside the tiled path (it spans more than
+// kMaxTiles tiles), with global atomics: the solid walk, or its surface
+// columns along x.
+template <bool SOLID>
+__device__ void direct_triangle(const VoxParams& p, const float* __restrict__ tris, uint64_t i,
+                                uint32_t* __restrict__ table) {
+    float t[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) t[k] = tris[i * 9 + k];
+    const Rows all{0, (int)p.G - 1, 0, (int)p.G - 1};
+    if constexpr (SOLID) {
+        GlobalSink<true> out{table};
+        solid_triangle<false>(p, t, &out, all);
+    } else {
+        const SurfTri st = surface_setup(p, t, all, true);
+        if (st.axis != 0) return;
+        GlobalSink<false> out{table};
+        const uint32_t cols = surface_columns<0>(st);
+        for (uint32_t j = 0; j < cols; ++j) surface_column<0>(p, st, j, out);
+        out.flush();
+    }
+}
+
+
 // ---- tiled path
 
 // counts[tile][block] = list entries of this block's triangles in the tile;
-// routes[i] = triangle i's packed tile range or marker.
+// routes[i] = triangle i's packed tile range or marker; misc[0] += triangles
+// of the direct route.
 // LDS: n_tiles counters, then the triangle staging buffer.
 // It also clears the table (zero_words > 0: a grid-stride share per block,
 // as fill_zero_kernel would).
@@ -710,6 +703,7 @@ __global__ __launch_bounds__(kVoxThreads) void tile_count_kernel(VoxParams p, co
                                                                  TileGeo geo, uint32_t n_blocks,
                                                                  uint32_t* __restrict__ counts,
                                                                  uint64_t* __restrict__ routes,
+                                                                 uint32_t* __restrict__ misc,
                                                                  uint32_t* __restrict__ table, uint64_t zero_words,
                                                                  int aligned16) {
     extern __shared__ uint32_t cnt[];
@@ -736,6 +730,7 @@ __global__ __launch_bounds__(kVoxThreads) void tile_count_kernel(VoxParams p, co
             const uint64_t rt = route_of(rows_of<SOLID>(p, t), geo);
             routes[base + threadIdx.x] = rt;
             if (rt < kRouteDirect) for_tiles(rt, geo, [&](uint32_t tile) { atomicAdd(&cnt[tile], 1u); });
+            else if (rt == kRouteDirect) atomicAdd(&misc[0], 1u);
         }
     }
     __syncthreads();
@@ -773,54 +768,59 @@ __global__ __launch_bounds__(kVoxThreads) void tile_scan_rows_kernel(uint32_t n_
     }
 }
 
-// One block: first slot of each tile's list, chunk numbering (chunk_start has
-// n_tiles + 1 entries; the last is the number of chunks).
-__global__ __launch_bounds__(kVoxThreads) void tile_scan_kernel(TileGeo geo, const uint32_t* __restrict__ totals,
-                                                                uint32_t* __restrict__ start,
-                                                                uint32_t* __restrict__ chunk_start) {
-    __shared__ uint32_t pr[kVoxThreads], pc[kVoxThreads];
-    const uint32_t n = geo.n_tiles;
-    const uint32_t per = (n + kVoxThreads - 1) / kVoxThreads;
-    const uint32_t b0 = min(n, threadIdx.x * per), b1 = min(n, b0 + per);
-    uint32_t sr = 0, sc = 0;
-    for (uint32_t i = b0; i < b1; ++i) {
-        sr += totals[i];
-        sc += (totals[i] + geo.chunk - 1) / geo.chunk;
-    }
-    pr[threadIdx.x] = sr;
-    pc[threadIdx.x] = sc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t ar = 0, ac = 0;
-        for (int i = 0; i < kVoxThreads; ++i) {
-            const uint32_t vr = pr[i], vc = pc[i];
-            pr[i] = ar;
-            pc[i] = ac;
-            ar += vr;
-            ac += vc;
-        }
-        chunk_start[n] = ac;
-    }
-    __syncthreads();
-    uint32_t ar = pr[threadIdx.x], ac = pc[threadIdx.x];
-    for (uint32_t i = b0; i < b1; ++i) {
-        start[i] = ar;
-        chunk_start[i] = ac;
-        ar += totals[i];
-        ac += (totals[i] + geo.chunk - 1) / geo.chunk;
-    }
-}
-
 // Writes the index of every tiled triangle into the lists of its tiles (in
-// the same block partition as the count pass).
+// the same block partition as the count pass).  Each block first scans the
+// tile totals itself for the first slot of each tile's list.  Block 0 also
+// stores those slots and the chunk numbering for the tile pass (chunk_start
+// has n_tiles + 1 entries; the last is the number of chunks), and moves the
+// count pass's direct-route triangles to misc[1], clearing misc[0] for the
+// next job.
 __global__ __launch_bounds__(kVoxThreads) void tile_scatter_kernel(uint64_t n_tri, TileGeo geo, uint32_t n_blocks,
                                                                    const uint64_t* __restrict__ routes,
                                                                    const uint32_t* __restrict__ row_off,
-                                                                   const uint32_t* __restrict__ start,
+                                                                   const uint32_t* __restrict__ totals,
+                                                                   uint32_t* __restrict__ start,
+                                                                   uint32_t* __restrict__ chunk_start,
+                                                                   uint32_t* __restrict__ misc,
                                                                    uint32_t* __restrict__ idx) {
     extern __shared__ uint32_t cursor[];
+    __shared__ uint32_t s_part[kVoxThreads / 64];
+    // exclusive scan of the totals, each thread a contiguous run of tiles
+    const uint32_t per = (geo.n_tiles + kVoxThreads - 1) / kVoxThreads;
+    const uint32_t t0 = min(geo.n_tiles, threadIdx.x * per), t1 = min(geo.n_tiles, t0 + per);
+    uint32_t sum = 0;
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t v = totals[t];
+        cursor[t] = v;
+        sum += v;
+    }
+    uint32_t excl;
+    block_exclusive_scan(sum, s_part, excl);
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t v = cursor[t];
+        cursor[t] = excl;
+        excl += v;
+    }
+    if (blockIdx.x == 0) {
+        auto chunks_of = [&](uint32_t t) { return (totals[t] + geo.chunk - 1) / geo.chunk; };
+        uint32_t csum = 0;
+        for (uint32_t t = t0; t < t1; ++t) csum += chunks_of(t);
+        uint32_t cex;
+        const uint32_t n_chunks = block_exclusive_scan(csum, s_part, cex);
+        for (uint32_t t = t0; t < t1; ++t) {
+            start[t] = cursor[t];
+            chunk_start[t] = cex;
+            cex += chunks_of(t);
+        }
+        if (threadIdx.x == 0) {
+            chunk_start[geo.n_tiles] = n_chunks;
+            misc[1] = misc[0];
+            misc[0] = 0;
+        }
+    }
+    __syncthreads();
     for (uint32_t t = threadIdx.x; t < geo.n_tiles; t += kVoxThreads)
-        cursor[t] = start[t] + row_off[(size_t)t * n_blocks + blockIdx.x];
+        cursor[t] += row_off[(size_t)t * n_blocks + blockIdx.x];
     __syncthreads();
     const uint64_t b0 = (uint64_t)blockIdx.x * geo.tris_per_block;
     const uint64_t b1 = min<uint64_t>(n_tri, b0 + geo.tris_per_block);
@@ -865,16 +865,21 @@ __device__ __forceinline__ void rows_suffix_xor(uint32_t* lds, uint32_t n_rows, 
 
 // Persistent: each iteration takes one chunk of one tile's list, runs the
 // per-triangle routine clipped to the tile's rows into an LDS copy of the
-// tile, then merges the nonzero words into the table.
+// tile, then merges the nonzero words into the table.  Then the triangles of
+// the direct route (misc[1] of them), with global atomics; while there are
+// any, every merge is an atomic.
 template <bool SOLID, bool DIFF>
 __global__ __launch_bounds__(kVoxThreads) void tile_vox_kernel(VoxParams p, const float* __restrict__ tris,
                                                                TileGeo geo, const uint32_t* __restrict__ totals,
                                                                const uint32_t* __restrict__ start,
                                                                const uint32_t* __restrict__ chunk_start,
                                                                const uint32_t* __restrict__ idx,
+                                                               const uint64_t* __restrict__ routes,
+                                                               const uint32_t* __restrict__ misc,
                                                                uint32_t* __restrict__ table) {
     extern __shared__ uint32_t lds[];
     const uint32_t n_chunks = chunk_start[geo.n_tiles];
+    const bool any_direct = misc[1] != 0;
     const uint32_t tile_words = geo.T * geo.wpp;
     const uint64_t GG = (uint64_t)p.G * p.G;
     for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
@@ -973,8 +978,9 @@ __global__ __launch_bounds__(kVoxThreads) void tile_vox_kernel(VoxParams p, cons
             rows_suffix_xor(lds, geo.T * geo.T, p.G / 32);
             __syncthreads();
         }
-        // a tile with one chunk that owns whole words is the only writer of them
-        const bool exclusive = geo.aligned && chunk_start[tile + 1] - chunk_start[tile] == 1;
+        // a tile with one chunk that owns whole words is the only writer of them,
+        // unless triangles of the direct route write into the table as well
+        const bool exclusive = geo.aligned && !any_direct && chunk_start[tile + 1] - chunk_start[tile] == 1;
         const uint32_t planes = (uint32_t)(clip.z1 - clip.z0 + 1);
         for (uint32_t k = threadIdx.x; k < planes * geo.wpp; k += kVoxThreads) {
             const uint32_t v = lds[k];
@@ -986,6 +992,11 @@ __global__ __launch_bounds__(kVoxThreads) void tile_vox_kernel(VoxParams p, cons
             else atomicOr(table + w, v);
         }
         __syncthreads();  // LDS is reused by the next chunk
+    }
+    if (any_direct) {
+        for (uint64_t i = (uint64_t)blockIdx.x * kVoxThreads + threadIdx.x; i < p.n_tri;
+             i += (uint64_t)gridDim.x * kVoxThreads)
+            if (routes[i] == kRouteDirect) direct_triangle<SOLID>(p, tris, i, table);
     }
 }
 
@@ -1131,7 +1142,14 @@ bool tile_geometry(uint32_t G, uint64_t n_tri, bool force, bool accumulate, Tile
     return true;
 }
 
-// Tiled-path scratch, one grow-only buffer per (device, stream).
+// The tiled path's scratch starts with kMiscWords job counters (misc[0]:
+// direct-route triangles, counted by the count pass and cleared by the
+// scatter pass, so it is zero between jobs; misc[1]: the same count handed to
+// the tile pass), at the same place for every job geometry.
+constexpr size_t kMiscWords = 16;
+
+// Tiled-path scratch, one grow-only buffer per (device, stream); a new
+// buffer's job counters are cleared on the stream before its first job.
 uint8_t* tile_workspace(hipStream_t s, size_t bytes) {
     static std::mutex mu;
     static std::map<std::pair<int, hipStream_t>, std::unique_ptr<DevBuf>> per;
@@ -1143,46 +1161,48 @@ uint8_t* tile_workspace(hipStream_t s, size_t bytes) {
     if (b->bytes < bytes) {
         NLOD_HIP(hipStreamSynchronize(s));  // earlier calls on this stream may still read it
         b->ensure(bytes);
+        hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(64), 0, s, b->as<uint32_t>(), (uint64_t)kMiscWords, 1);
+        NLOD_LAUNCH_CHECK();
     }
     return b->as<uint8_t>();
 }
 
 template <bool SOLID>
-void launch_direct(hipStream_t stream, const VoxParams& p, const float* d_tris, const uint64_t* routes,
-                   uint32_t* d_table) {
+void launch_direct(hipStream_t stream, const VoxParams& p, const float* d_tris, uint32_t* d_table) {
     const uint64_t blocks = std::min<uint64_t>((p.n_tri + kVoxThreads - 1) / kVoxThreads, 1u << 20);
     if (SOLID)
-        hipLaunchKernelGGL(vox_solid_kernel, dim3((uint32_t)blocks), dim3(kVoxThreads), 0, stream, p, d_tris, routes,
-                           d_table);
+        hipLaunchKernelGGL(vox_solid_kernel, dim3((uint32_t)blocks), dim3(kVoxThreads), 0, stream, p, d_tris, d_table);
     else
         hipLaunchKernelGGL(vox_surface_kernel, dim3((uint32_t)blocks), dim3(kVoxThreads), 0, stream, p, d_tris,
-                           routes, d_table);
+                           d_table);
     NLOD_LAUNCH_CHECK();
 }
 
-// Bytes of tiled-path scratch for a job of this geometry: routes (u64 per
-// triangle) | counts | totals | start | chunk_start (n_tiles + 1) | idx.
+// Bytes of tiled-path scratch for a job of this geometry: misc | routes (u64
+// per triangle) | counts | totals | start | chunk_start (n_tiles + 1) | idx.
 size_t tile_scratch_bytes(const VoxParams& p, const TileGeo& geo) {
     const uint32_t n_blocks = (uint32_t)((p.n_tri + geo.tris_per_block - 1) / geo.tris_per_block);
     const size_t n_counts = (size_t)geo.n_tiles * n_blocks;
     const size_t n_idx = (size_t)p.n_tri * kMaxTiles;
-    return (2 * (size_t)p.n_tri + n_counts + 3 * (size_t)geo.n_tiles + 1 + n_idx) * sizeof(uint32_t);
+    return (kMiscWords + 2 * (size_t)p.n_tri + n_counts + 3 * (size_t)geo.n_tiles + 1 + n_idx) * sizeof(uint32_t);
 }
 
 // The tiled path's scratch comes from `workspace(bytes)`: the library's
-// per-stream buffer, or the caller-free static array of nlod_voxelize_noalloc.
-// Every word of it that a kernel reads was written earlier in the same job.
+// per-stream buffer, or the caller-free static array of nlod_voxelize_noalloc
+// (zero at load).  Every word of it that a kernel reads was written earlier in
+// the same job, except misc[0], which jobs leave zero.
 template <bool SOLID, typename Workspace>
 void launch_voxelize(hipStream_t stream, const VoxParams& p, const TileGeo& geo, bool tiled, const float* d_tris,
                      uint32_t* d_table, Workspace&& workspace) {
     if (!tiled) {
-        launch_direct<SOLID>(stream, p, d_tris, nullptr, d_table);
+        launch_direct<SOLID>(stream, p, d_tris, d_table);
         return;
     }
     const uint32_t n_blocks = (uint32_t)((p.n_tri + geo.tris_per_block - 1) / geo.tris_per_block);
     const size_t n_counts = (size_t)geo.n_tiles * n_blocks;
     uint8_t* ws = workspace(tile_scratch_bytes(p, geo));
-    uint64_t* routes = reinterpret_cast<uint64_t*>(ws);
+    uint32_t* misc = reinterpret_cast<uint32_t*>(ws);
+    uint64_t* routes = reinterpret_cast<uint64_t*>(misc + kMiscWords);
     uint32_t* counts = reinterpret_cast<uint32_t*>(routes + p.n_tri);
     uint32_t* totals = counts + n_counts;
     uint32_t* start = totals + geo.n_tiles;
@@ -1194,15 +1214,14 @@ void launch_voxelize(hipStream_t stream, const VoxParams& p, const TileGeo& geo,
     (void)occupancy_blocks(reinterpret_cast<const void*>(tile_scatter_kernel), kVoxThreads, scatter_smem);
     const uint64_t zero_words = geo.accumulate ? 0 : ((uint64_t)p.G * p.G * p.G + 31) / 32;
     hipLaunchKernelGGL(tile_count_kernel<SOLID>, dim3(n_blocks), dim3(kVoxThreads), count_smem, stream, p, d_tris,
-                       geo, n_blocks, counts, routes, d_table, zero_words, (int)(((uintptr_t)d_table & 15u) == 0));
+                       geo, n_blocks, counts, routes, misc, d_table, zero_words,
+                       (int)(((uintptr_t)d_table & 15u) == 0));
     NLOD_LAUNCH_CHECK();
     hipLaunchKernelGGL(tile_scan_rows_kernel, dim3(geo.n_tiles), dim3(kVoxThreads), 0, stream, n_blocks, counts,
                        totals);
     NLOD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(kVoxThreads), 0, stream, geo, totals, start, chunk_start);
-    NLOD_LAUNCH_CHECK();
     hipLaunchKernelGGL(tile_scatter_kernel, dim3(n_blocks), dim3(kVoxThreads), scatter_smem, stream, p.n_tri, geo,
-                       n_blocks, routes, counts, start, idx);
+                       n_blocks, routes, counts, totals, start, chunk_start, misc, idx);
     NLOD_LAUNCH_CHECK();
     size_t tile_smem = (size_t)geo.T * geo.wpp * sizeof(uint32_t) +
                        (kVoxThreads + 1 + kVoxThreads / 64) * sizeof(uint32_t);
@@ -1211,13 +1230,11 @@ void launch_voxelize(hipStream_t stream, const VoxParams& p, const TileGeo& geo,
         const int per_cu = occupancy_blocks(reinterpret_cast<const void*>(kernel), kVoxThreads, tile_smem);
         const uint32_t grid = (uint32_t)(cu_count() * per_cu);
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(kVoxThreads), tile_smem, stream, p, d_tris, geo, totals, start,
-                           chunk_start, idx, d_table);
+                           chunk_start, idx, routes, misc, d_table);
         NLOD_LAUNCH_CHECK();
     };
     if (SOLID && geo.aligned) run_tiles(tile_vox_kernel<SOLID, true>);
     else run_tiles(tile_vox_kernel<SOLID, false>);
-    // the triangles spanning more than kMaxTiles tiles
-    launch_direct<SOLID>(stream, p, d_tris, routes, d_table);
 }
 
 void validate_info(const nlod_voxinfo* info, uint32_t flags) {
