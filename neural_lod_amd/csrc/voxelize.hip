@@ -742,25 +742,15 @@ __global__ __launch_bounds__(kVoxThreads) void tile_count_kernel(VoxParams p, co
 // place, and the row total.
 __global__ __launch_bounds__(kVoxThreads) void tile_scan_rows_kernel(uint32_t n_blocks, uint32_t* __restrict__ counts,
                                                                      uint32_t* __restrict__ totals) {
-    __shared__ uint32_t part[kVoxThreads];
+    __shared__ uint32_t part[kVoxThreads / 64];
     uint32_t* row = counts + (size_t)blockIdx.x * n_blocks;
     const uint32_t per = (n_blocks + kVoxThreads - 1) / kVoxThreads;
     const uint32_t b0 = min(n_blocks, threadIdx.x * per), b1 = min(n_blocks, b0 + per);
     uint32_t s = 0;
     for (uint32_t i = b0; i < b1; ++i) s += row[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int i = 0; i < kVoxThreads; ++i) {
-            const uint32_t v = part[i];
-            part[i] = acc;
-            acc += v;
-        }
-        totals[blockIdx.x] = acc;
-    }
-    __syncthreads();
-    uint32_t acc = part[threadIdx.x];
+    uint32_t acc;
+    const uint32_t total = block_exclusive_scan(s, part, acc);
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
     for (uint32_t i = b0; i < b1; ++i) {
         const uint32_t v = row[i];
         row[i] = acc;
@@ -895,6 +885,18 @@ __global__ __launch_bounds__(kVoxThreads) void tile_vox_kernel(VoxParams p, cons
                         min((tz + 1) * (int)geo.T, (int)p.G) - 1};
         const uint32_t begin = start[tile] + (c - chunk_start[tile]) * geo.chunk;
         const uint32_t end = min(start[tile] + totals[tile], begin + geo.chunk);
+        // each lane's triangle of the batch at kb; the next batch's is loaded
+        // while the current one runs (the first while the tile is cleared)
+        auto load_tri = [&](uint32_t kb, float(&t)[9]) {
+            const uint32_t k = kb + threadIdx.x;
+            if (k >= end) return false;
+            const uint64_t i = idx[k];
+#pragma unroll
+            for (int j = 0; j < 9; ++j) t[j] = tris[i * 9 + j];
+            return true;
+        };
+        float tn[9];
+        bool hn = load_tri(begin, tn);
         for (uint32_t k = threadIdx.x; k < tile_words; k += kVoxThreads) lds[k] = 0;
         __syncthreads();
         if constexpr (SOLID) {
@@ -908,13 +910,13 @@ __global__ __launch_bounds__(kVoxThreads) void tile_vox_kernel(VoxParams p, cons
             uint32_t* s_part = s_off + kVoxThreads + 1;
             LdsSink<true, DIFF> out{lds, (uint64_t)p.G, clip.z0, clip.y0, geo.wpp};
             for (uint32_t kb = begin; kb < end; kb += kVoxThreads) {
-                const uint32_t k = kb + threadIdx.x;
-                uint32_t cnt = 0;
-                if (k < end) {
-                    const uint64_t i = idx[k];
-                    float t[9];
+                float t[9];
 #pragma unroll
-                    for (int j = 0; j < 9; ++j) t[j] = tris[i * 9 + j];
+                for (int j = 0; j < 9; ++j) t[j] = tn[j];
+                const bool have = hn;
+                hn = load_tri(kb + kVoxThreads, tn);
+                uint32_t cnt = 0;
+                if (have) {
                     SolidTri st;
                     if (solid_setup(p, t, st)) {
                         st.y0 = max(st.y0, clip.y0);
@@ -955,16 +957,14 @@ __global__ __launch_bounds__(kVoxThreads) void tile_vox_kernel(VoxParams p, cons
             uint32_t* s_pre = reinterpret_cast<uint32_t*>(s_st + kVoxThreads);
             uint32_t* s_part = s_pre + kVoxThreads + 1;
             for (uint32_t kb = begin; kb < end; kb += kVoxThreads) {
-                const uint32_t k = kb + threadIdx.x;
+                float t[9];
+#pragma unroll
+                for (int j = 0; j < 9; ++j) t[j] = tn[j];
+                const bool have = hn;
+                hn = load_tri(kb + kVoxThreads, tn);
                 SurfTri st;
                 st.axis = -1;
-                if (k < end) {
-                    const uint64_t i = idx[k];
-                    float t[9];
-#pragma unroll
-                    for (int j = 0; j < 9; ++j) t[j] = tris[i * 9 + j];
-                    st = surface_setup(p, t, clip);
-                }
+                if (have) st = surface_setup(p, t, clip);
                 s_st[threadIdx.x] = st;
                 __syncthreads();
                 LdsSink<false> out{lds, (uint64_t)p.G, clip.z0, clip.y0, geo.wpp};
@@ -1167,9 +1167,17 @@ uint8_t* tile_workspace(hipStream_t s, size_t bytes) {
     return b->as<uint8_t>();
 }
 
+// The direct kernels loop over 256-triangle batches.  While the batches fit
+// in four rounds of two blocks per CU, the grid stays at two blocks per CU:
+// three resident blocks per CU (their occupancy) would finish a batch count
+// just above one round with a tail of single blocks (200k triangles: 782
+// batches against 768 slots).
 template <bool SOLID>
 void launch_direct(hipStream_t stream, const VoxParams& p, const float* d_tris, uint32_t* d_table) {
-    const uint64_t blocks = std::min<uint64_t>((p.n_tri + kVoxThreads - 1) / kVoxThreads, 1u << 20);
+    const uint64_t batches = (p.n_tri + kVoxThreads - 1) / kVoxThreads;
+    const uint64_t pair_grid = 2ull * (uint64_t)cu_count();
+    const uint64_t blocks = std::min<uint64_t>(batches <= 4 * pair_grid ? std::min(batches, pair_grid) : batches,
+                                               1u << 20);
     if (SOLID)
         hipLaunchKernelGGL(vox_solid_kernel, dim3((uint32_t)blocks), dim3(kVoxThreads), 0, stream, p, d_tris, d_table);
     else
