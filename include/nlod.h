@@ -107,8 +107,8 @@ int nlod_voxelize_mesh(void* stream, const float* d_triangles, uint64_t n_triang
  * flags and bits as nlod_voxelize; the LDS-tiled path takes its scratch from a
  * fixed device array of 32 MiB per GPU (held until the process ends) instead
  * of a library allocation.  A job whose tiled path needs more scratch than
- * that fails with NLOD_ERR_INVALID_ARGUMENT (about 550k triangles at 256^3,
- * 200k at 1024^3); NLOD_VOX_DIRECT needs no scratch at any size.  Every call
+ * that fails with NLOD_ERR_INVALID_ARGUMENT (about 700k triangles at 256^3,
+ * 300k at 1024^3); NLOD_VOX_DIRECT needs no scratch at any size.  Every call
  * on one GPU shares its array: calls that overlap on one GPU (on different
  * streams or threads) race and must be serialized by the caller.
  * Asynchronous. */

@@ -900,14 +900,16 @@ __global__ __launch_bounds__(kVoxThreads) void tile_vox_kernel(VoxParams p, cons
         for (uint32_t k = threadIdx.x; k < tile_words; k += kVoxThreads) lds[k] = 0;
         __syncthreads();
         if constexpr (SOLID) {
-            // A batch of kVoxThreads triangles: each lane sets up one and counts its
-            // sample rows inside the tile; a block scan numbers all the samples, and
-            // lanes then take samples in turn from any triangle of the batch, so a
-            // wave runs the batch's mean sample count, not its largest triangle's.
-            // After the tile: set-up triangles, sample offsets, scan partials.
-            SolidTri* s_st = reinterpret_cast<SolidTri*>(lds + tile_words);
-            uint32_t* s_off = reinterpret_cast<uint32_t*>(s_st + kVoxThreads);  // kVoxThreads + 1
-            uint32_t* s_part = s_off + kVoxThreads + 1;
+            // A batch of kVoxThreads triangles, 64 per wave: each lane sets up one
+            // and counts its sample rows inside the tile; a wave scan numbers the
+            // wave's samples, and its lanes then take samples in turn from any of
+            // its triangles, so a wave runs its mean sample count, not its largest
+            // triangle's, and no block barrier separates the batches.  After the
+            // tile: set-up triangles (64 per wave), then 65 sample offsets per wave.
+            const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+            SolidTri* s_st_all = reinterpret_cast<SolidTri*>(lds + tile_words);
+            SolidTri* s_st = s_st_all + wave * 64;
+            uint32_t* s_off = reinterpret_cast<uint32_t*>(s_st_all + kVoxThreads) + wave * 65;
             LdsSink<true, DIFF> out{lds, (uint64_t)p.G, clip.z0, clip.y0, geo.wpp};
             for (uint32_t kb = begin; kb < end; kb += kVoxThreads) {
                 float t[9];
@@ -925,17 +927,25 @@ __global__ __launch_bounds__(kVoxThreads) void tile_vox_kernel(VoxParams p, cons
                         st.z1 = min(st.z1, clip.z1);
                         if (st.y0 <= st.y1 && st.z0 <= st.z1) {
                             cnt = (uint32_t)(st.y1 - st.y0 + 1) * (uint32_t)(st.z1 - st.z0 + 1);
-                            s_st[threadIdx.x] = st;
+                            s_st[lane] = st;
                         }
                     }
                 }
-                uint32_t excl;
-                const uint32_t total = block_exclusive_scan(cnt, s_part, excl);
-                s_off[threadIdx.x] = excl;
-                if (threadIdx.x == 0) s_off[kVoxThreads] = total;
-                __syncthreads();
-                for (uint32_t si = threadIdx.x; si < total; si += kVoxThreads) {
-                    uint32_t lo = 0, hi = kVoxThreads;  // s_off[lo] <= si < s_off[hi]
+                uint32_t x = cnt;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const uint32_t y = (uint32_t)__shfl_up((int)x, off, 64);
+                    if (lane >= (uint32_t)off) x += y;
+                }
+                const uint32_t total = (uint32_t)__shfl((int)x, 63, 64);
+                s_off[lane] = x - cnt;
+                if (lane == 63) s_off[64] = total;
+                // the wave's own LDS writes, read back by other lanes of the wave
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (uint32_t si = lane; si < total; si += 64) {
+                    uint32_t lo = 0, hi = 64;  // s_off[lo] <= si < s_off[hi]
                     while (hi - lo > 1) {
                         const uint32_t mid = (lo + hi) >> 1;
                         if (s_off[mid] <= si) lo = mid;
@@ -944,12 +954,15 @@ __global__ __launch_bounds__(kVoxThreads) void tile_vox_kernel(VoxParams p, cons
                     const SolidTri st = s_st[lo];
                     const uint32_t local = si - s_off[lo];
                     const uint32_t nz = (uint32_t)(st.z1 - st.z0 + 1);
-                    // local / nz: local < 64 nz and nz <= 64 (tiles are at most 22 rows
+                    // local / nz: local < 64 nz and nz <= 64 (tiles are at most 64 rows
                     // a side), where a float reciprocal within 2 ulp floors exactly
                     const uint32_t yi = (uint32_t)(((float)local + 0.5f) * __frcp_rn((float)nz));
                     solid_sample(p, st, st.y0 + (int)yi, st.z0 + (int)(local - yi * nz), &out);
                 }
-                __syncthreads();  // the batch's set-ups and offsets are reused by the next
+                // the wave's set-ups and offsets are reused by its next batch
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
         } else {
             // after the tile: set-up triangles, column prefix, scan partials
