@@ -70,6 +70,69 @@ __host__ __device__ inline uint32_t permuted_unit(uint32_t c, uint32_t g, uint32
     return 32u * c + (j < 4 ? 4u * g + j : 16u + 4u * g + (j - 4u));
 }
 
+// Packed transposed weights for the backward products (same fragment order as
+// the forward packing: per matrix, per 16-row tile, per 32-column chunk).
+//   output layer: W_out^T [W][32]  (columns = outputs, rows >= 16 zero)
+//   hidden l (L-1 .. 1): W_l^T [W][W]
+//   input layer: W_0^T [Kp][W]  (rows >= K zero)
+// Columns are permuted to the accumulator layout of the delta they multiply.
+inline size_t packed_bwd_halves(const NetShape& n) {
+    const uint32_t W = n.width, Kp = k_padded_of(n);
+    const size_t frags = frag_count(W, 32) + (size_t)(n.n_hidden_layers - 1) * frag_count(W, W) + frag_count(Kp, W);
+    return frags * 512;
+}
+
+// Element e of the forward packing of the weight vector w (W wide, K inputs
+// padded to Kp, L hidden layers).
+__device__ inline float packed_fwd_value(uint32_t W, uint32_t K, uint32_t Kp, uint32_t L, const float* __restrict__ w,
+                                         uint32_t e) {
+    const uint32_t j = e & 7u, lane = (e >> 3) & 63u, frag = e >> 9;
+    const uint32_t g = lane >> 4, r16 = lane & 15u;
+    const uint32_t f0 = (W / 16) * (Kp / 32);
+    const uint32_t fh = (W / 16) * (W / 32);
+    if (frag < f0) {  // input layer: natural column order, zero beyond K
+        const uint32_t mt = frag / (Kp / 32), c = frag % (Kp / 32);
+        const uint32_t row = 16 * mt + r16, col = 32 * c + 8 * g + j;
+        return col < K ? w[(size_t)row * K + col] : 0.0f;
+    }
+    if (frag < f0 + (L - 1) * fh) {  // hidden matrices: permuted columns
+        const uint32_t f = frag - f0, layer = 1 + f / fh, ff = f % fh;
+        const uint32_t mt = ff / (W / 32), c = ff % (W / 32);
+        const uint32_t row = 16 * mt + r16, col = permuted_unit(c, g, j);
+        return w[(size_t)W * K + (size_t)(layer - 1) * W * W + (size_t)row * W + col];
+    }
+    // output matrix (16 padded rows): permuted columns
+    const uint32_t c = frag - f0 - (L - 1) * fh;
+    const uint32_t row = r16, col = permuted_unit(c, g, j);
+    return w[(size_t)W * K + (size_t)(L - 1) * W * W + (size_t)row * W + col];
+}
+
+// Element e of the backward packing of the weight vector w.
+__device__ inline float packed_bwd_value(uint32_t W, uint32_t K, uint32_t L, const float* __restrict__ w, uint32_t e) {
+    const uint32_t j = e & 7u, lane = (e >> 3) & 63u, frag = e >> 9;
+    const uint32_t g = lane >> 4, r16 = lane & 15u;
+    const uint32_t f_out = W / 16;  // output-layer fragments (1 chunk each)
+    const uint32_t f_hid = (W / 16) * (W / 32);
+    const size_t off_out = (size_t)W * K + (size_t)(L - 1) * W * W;
+    if (frag < f_out) {
+        const uint32_t mt = frag;
+        const uint32_t row = 16 * mt + r16;           // input unit of W_out
+        const uint32_t col = permuted_unit(0, g, j);  // output index
+        return col < 16 ? w[off_out + (size_t)col * W + row] : 0.0f;
+    }
+    if (frag < f_out + (L - 1) * f_hid) {
+        const uint32_t f = frag - f_out, idx = f / f_hid, ff = f % f_hid;
+        const uint32_t layer = L - 1 - idx;  // hidden matrices in backward order
+        const uint32_t mt = ff / (W / 32), c = ff % (W / 32);
+        const uint32_t row = 16 * mt + r16, col = permuted_unit(c, g, j);
+        return w[(size_t)W * K + (size_t)(layer - 1) * W * W + (size_t)col * W + row];
+    }
+    const uint32_t f = frag - f_out - (L - 1) * f_hid;
+    const uint32_t mt = f / (W / 32), c = f % (W / 32);
+    const uint32_t row = 16 * mt + r16, col = permuted_unit(c, g, j);
+    return row < K ? w[(size_t)col * K + row] : 0.0f;
+}
+
 // Table entry of F features (FP32 or FP16 storage) as floats.
 template <int F, typename TT>
 __device__ __forceinline__ void load_entry(const TT* __restrict__ p, float* out) {

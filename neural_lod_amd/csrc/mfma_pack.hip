@@ -12,30 +12,14 @@
 namespace nlod {
 namespace {
 
+// Elements [0, total_fwd) of the forward packing, then (out_bwd non-null) the
+// backward packing, in one launch.
 __global__ void pack_kernel(uint32_t W, uint32_t K, uint32_t Kp, uint32_t L, const float* __restrict__ w,
-                            half_t* __restrict__ out, uint32_t total_halves) {
+                            half_t* __restrict__ out_fwd, uint32_t total_fwd, half_t* __restrict__ out_bwd,
+                            uint32_t total_bwd) {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= total_halves) return;
-    const uint32_t j = e & 7u, lane = (e >> 3) & 63u, frag = e >> 9;
-    const uint32_t g = lane >> 4, r16 = lane & 15u;
-    const uint32_t f0 = (W / 16) * (Kp / 32);
-    const uint32_t fh = (W / 16) * (W / 32);
-    float v = 0.0f;
-    if (frag < f0) {  // input layer: natural column order, zero beyond K
-        const uint32_t mt = frag / (Kp / 32), c = frag % (Kp / 32);
-        const uint32_t row = 16 * mt + r16, col = 32 * c + 8 * g + j;
-        v = col < K ? w[(size_t)row * K + col] : 0.0f;
-    } else if (frag < f0 + (L - 1) * fh) {  // hidden matrices: permuted columns
-        const uint32_t f = frag - f0, layer = 1 + f / fh, ff = f % fh;
-        const uint32_t mt = ff / (W / 32), c = ff % (W / 32);
-        const uint32_t row = 16 * mt + r16, col = permuted_unit(c, g, j);
-        v = w[(size_t)W * K + (size_t)(layer - 1) * W * W + (size_t)row * W + col];
-    } else {  // output matrix (16 padded rows): permuted columns
-        const uint32_t c = frag - f0 - (L - 1) * fh;
-        const uint32_t row = r16, col = permuted_unit(c, g, j);
-        v = w[(size_t)W * K + (size_t)(L - 1) * W * W + (size_t)row * W + col];
-    }
-    out[e] = (half_t)v;
+    if (e < total_fwd) out_fwd[e] = (half_t)packed_fwd_value(W, K, Kp, L, w, e);
+    else if (out_bwd && e - total_fwd < total_bwd) out_bwd[e - total_fwd] = (half_t)packed_bwd_value(W, K, L, w, e - total_fwd);
 }
 
 EncProgram build_program(const Layout& L) {
@@ -101,10 +85,13 @@ void mfma_upload_program(const Layout& L, void* d_program) {
     NLOD_HIP(hipMemcpy(d_program, &host, sizeof(EncProgram), hipMemcpyHostToDevice));
 }
 
-void mfma_pack_weights(hipStream_t stream, const NetShape& net, const float* weights, void* packed) {
-    const uint32_t total = (uint32_t)packed_halves(net);
-    hipLaunchKernelGGL(pack_kernel, dim3(div_round_up(total, 256)), dim3(256), 0, stream, net.width, net.n_input,
-                       k_padded_of(net), net.n_hidden_layers, weights, static_cast<half_t*>(packed), total);
+void mfma_pack_weights(hipStream_t stream, const NetShape& net, const float* weights, void* packed,
+                       void* packed_bwd) {
+    const uint32_t total_fwd = (uint32_t)packed_halves(net);
+    const uint32_t total_bwd = packed_bwd ? (uint32_t)packed_bwd_halves(net) : 0u;
+    hipLaunchKernelGGL(pack_kernel, dim3(div_round_up(total_fwd + total_bwd, 256)), dim3(256), 0, stream, net.width,
+                       net.n_input, k_padded_of(net), net.n_hidden_layers, weights, static_cast<half_t*>(packed),
+                       total_fwd, static_cast<half_t*>(packed_bwd), total_bwd);
     NLOD_LAUNCH_CHECK();
 }
 

@@ -117,48 +117,6 @@ __global__ __launch_bounds__(kBlock) void mlp_fwd_train_kernel(TrainArgs a, cons
     }
 }
 
-// Packed transposed weights for the backward products (same fragment order as
-// the forward packing: per matrix, per 16-row tile, per 32-column chunk).
-//   output layer: W_out^T [W][32]  (columns = outputs, rows >= 16 zero)
-//   hidden l (L-1 .. 1): W_l^T [W][W]
-//   input layer: W_0^T [Kp][W]  (rows >= K zero)
-// Columns are permuted to the accumulator layout of the delta they multiply.
-size_t packed_bwd_halves(const NetShape& n) {
-    const uint32_t W = n.width, Kp = k_padded_of(n);
-    const size_t frags = frag_count(W, 32) + (size_t)(n.n_hidden_layers - 1) * frag_count(W, W) + frag_count(Kp, W);
-    return frags * 512;
-}
-
-__global__ void pack_bwd_kernel(uint32_t W, uint32_t K, uint32_t Kp, uint32_t L,
-                                const float* __restrict__ w, half_t* __restrict__ out, uint32_t total) {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= total) return;
-    const uint32_t j = e & 7u, lane = (e >> 3) & 63u, frag = e >> 9;
-    const uint32_t g = lane >> 4, r16 = lane & 15u;
-    const uint32_t f_out = W / 16;             // output-layer fragments (1 chunk each)
-    const uint32_t f_hid = (W / 16) * (W / 32);
-    const size_t off_out = (size_t)W * K + (size_t)(L - 1) * W * W;
-    float v = 0.0f;
-    if (frag < f_out) {
-        const uint32_t mt = frag;
-        const uint32_t row = 16 * mt + r16;            // input unit of W_out
-        const uint32_t col = permuted_unit(0, g, j);   // output index
-        v = col < 16 ? w[off_out + (size_t)col * W + row] : 0.0f;
-    } else if (frag < f_out + (L - 1) * f_hid) {
-        const uint32_t f = frag - f_out, idx = f / f_hid, ff = f % f_hid;
-        const uint32_t layer = L - 1 - idx;  // hidden matrices in backward order
-        const uint32_t mt = ff / (W / 32), c = ff % (W / 32);
-        const uint32_t row = 16 * mt + r16, col = permuted_unit(c, g, j);
-        v = w[(size_t)W * K + (size_t)(layer - 1) * W * W + (size_t)col * W + row];
-    } else {
-        const uint32_t f = frag - f_out - (L - 1) * f_hid;
-        const uint32_t mt = f / (W / 32), c = f % (W / 32);
-        const uint32_t row = 16 * mt + r16, col = permuted_unit(c, g, j);
-        v = row < K ? w[(size_t)col * K + row] : 0.0f;
-    }
-    out[e] = (half_t)v;
-}
-
 template <int W>
 __global__ __launch_bounds__(kBlock) void mlp_bwd_kernel(TrainArgs a, const half_t* __restrict__ packed_bwd,
                                                          const half_t* __restrict__ H, const float* __restrict__ y,
@@ -366,13 +324,6 @@ uint32_t persistent_blocks(Kern kern, size_t smem, uint32_t n_tiles) {
 
 size_t mfma_packed_bwd_bytes(const NetShape& net) { return packed_bwd_halves(net) * sizeof(half_t); }
 
-void mfma_pack_bwd_weights(hipStream_t stream, const NetShape& net, const float* weights, void* packed) {
-    const uint32_t total = (uint32_t)packed_bwd_halves(net);
-    hipLaunchKernelGGL(pack_bwd_kernel, dim3(div_round_up(total, 256)), dim3(256), 0, stream, net.width, net.n_input,
-                       k_padded_of(net), net.n_hidden_layers, weights,
-                       static_cast<half_t*>(packed), total);
-    NLOD_LAUNCH_CHECK();
-}
 
 size_t mfma_hidden_bytes(const Layout& L, uint32_t N) {
     return sizeof(half_t) * (size_t)L.net.n_hidden_layers * div_round_up(N, 16) * L.net.width * 16;

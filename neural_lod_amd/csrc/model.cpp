@@ -54,12 +54,10 @@ void Model::set_params(hipStream_t s, const float* p, const float* p_inf) {
 void Model::refresh_packed(hipStream_t s, bool training, bool inference) {
     if (precision != PREC_FP16) return;
     if (training) {
-        mfma_pack_weights(s, L.net, params, packed.p);
-        mfma_pack_bwd_weights(s, L.net, params, packed_bwd.p);
+        mfma_pack_weights(s, L.net, params, packed.p, packed_bwd.p);
     }
     if (inference && !inference_aliased) {
-        mfma_pack_weights(s, L.net, params_inference, packed_inference.p);
-        mfma_pack_bwd_weights(s, L.net, params_inference, packed_bwd_inference.p);
+        mfma_pack_weights(s, L.net, params_inference, packed_inference.p, packed_bwd_inference.p);
     }
 }
 

@@ -119,7 +119,10 @@ size_t mfma_program_bytes();
 size_t mfma_fused_smem_bytes(const NetShape& net);
 // encoding description for the fused kernels, copied to device memory (synchronous)
 void mfma_upload_program(const Layout& L, void* d_program);
-void mfma_pack_weights(hipStream_t stream, const NetShape& net, const float* weights, void* packed);
+// FP16 weight fragments of the forward packing and (packed_bwd non-null) the
+// backward packing of the transposed weights, in one launch
+void mfma_pack_weights(hipStream_t stream, const NetShape& net, const float* weights, void* packed,
+                       void* packed_bwd);
 // FP32 -> FP16 copy (the grid tables read by the encoding pass)
 void mfma_to_half(hipStream_t stream, const float* src, void* dst, size_t n);
 // bytes of the FP16 feature buffer for N queries ([N/16][K/8][16][8] halves)
@@ -138,7 +141,6 @@ void mfma_mlp_forward(hipStream_t stream, const Layout& L, const void* packed, c
 
 // ---- mlp_mfma_train.hip (FP16 MFMA training kernels) ------------------------------
 size_t mfma_packed_bwd_bytes(const NetShape& net);
-void mfma_pack_bwd_weights(hipStream_t stream, const NetShape& net, const float* weights, void* packed);
 // saved hidden activations: half [L][N/16][W][16]
 size_t mfma_hidden_bytes(const Layout& L, uint32_t N);
 // pre-activation gradients: half [L][N/16][W][16] + output layer [N/16][16][16]
