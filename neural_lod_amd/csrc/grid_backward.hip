@@ -374,6 +374,19 @@ __device__ __forceinline__ int fixed_shift(uint32_t max_bits) {
 
 __device__ __forceinline__ bool finite_bits(uint32_t b) { return b < 0x7f800000u; }
 
+// c * 2^S rounded to the nearest integer (ties to even), as two's-complement
+// int64: the rounded value r splits exactly into r = hi * 2^32 + lo with hi =
+// floor(r / 2^32) in [-2^30, 2^30) (|r| < 2^62) and lo in [0, 2^32), both
+// integral floats that convert exactly; the pair is the int64's two words.
+__device__ __forceinline__ unsigned long long to_fixed(float c, int S) {
+    const float r = __builtin_rintf(ldexpf(c, S));
+    const float hi = floorf(r * 0x1p-32f);
+    const float lo = fmaf(hi, -0x1p32f, r);
+    const uint32_t w0 = (uint32_t)lo;
+    const uint32_t w1 = (uint32_t)(int32_t)hi;
+    return ((unsigned long long)w1 << 32) | w0;
+}
+
 // Pass 3: group gi of its tile.  The tile's records are its runs in
 // (repetition, query block) order: run j holds run_pre[j + 1] - run_pre[j]
 // records from slot run_addr[j].  Each wave walks a contiguous quarter of the
@@ -453,7 +466,7 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(uint32_t n_bins, uint32_
 #pragma unroll
         for (uint32_t k = 0; k < kInFlight; ++k) load_rec<RW>(recs + at[k] * RW, dst[k]);
     };
-    auto fixed = [&](float c) { return (unsigned long long)__float2ll_rn(ldexpf(c, S)); };
+    auto fixed = [&](float c) { return to_fixed(c, S); };
     if (w0 < w1) load_batch(w0, cur);
     for (uint32_t i0 = w0; i0 < w1; i0 += 64 * kInFlight) {
         const uint32_t i1 = i0 + 64 * kInFlight;
