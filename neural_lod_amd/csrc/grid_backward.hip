@@ -126,6 +126,29 @@ __device__ __forceinline__ void edges(const GridBwdRow& r, const float* x, Emit&
     }
 }
 
+// Exclusive prefix sum of v over the block (wave shuffles, then the wave
+// totals through s_wave); returns the block's total.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_wave, uint32_t& excl) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        const uint32_t t = s_wave[w];
+        if (w < (int)wave) before += t;
+        total += t;
+    }
+    excl = before + x - v;
+    return total;
+}
+
 __global__ void reset_meta_kernel(uint32_t* meta) {
     if (threadIdx.x < 4) meta[threadIdx.x] = 0;
 }
@@ -135,7 +158,7 @@ __global__ void reset_meta_kernel(uint32_t* meta) {
 __global__ __launch_bounds__(kBlock) void scan_rows_kernel(uint32_t n_qblocks, const GridBwdBin* __restrict__ bins,
                                                            uint32_t* __restrict__ counts,
                                                            uint32_t* __restrict__ totals) {
-    __shared__ uint32_t part[kBlock];
+    __shared__ uint32_t s_wave[kBlock / 64];
     const GridBwdBin bin = bins[blockIdx.x];
     uint32_t* row = counts + (size_t)bin.count_row * n_qblocks;
     const uint32_t len = bin.reps * n_qblocks;
@@ -144,19 +167,9 @@ __global__ __launch_bounds__(kBlock) void scan_rows_kernel(uint32_t n_qblocks, c
     const uint32_t b0 = min(len, threadIdx.x * per), b1 = min(len, b0 + per);
     uint32_t s = 0;
     for (uint32_t i = b0; i < b1; ++i) s += row[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int i = 0; i < kBlock; ++i) {
-            const uint32_t v = part[i];
-            part[i] = acc;
-            acc += v;
-        }
-        totals[blockIdx.x] = acc;
-    }
-    __syncthreads();
-    uint32_t acc = part[threadIdx.x];
+    uint32_t acc;
+    const uint32_t total = block_exclusive_scan(s, s_wave, acc);
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
     for (uint32_t i = b0; i < b1; ++i) {
         const uint32_t v = row[i];
         row[i] = acc;
@@ -265,7 +278,7 @@ __global__ __launch_bounds__(kBlock) void bin_kernel(uint32_t N, uint32_t ld, ui
     constexpr uint32_t E = kTileFloats / F;
     constexpr int RW = RecWords<F>::value;
     __shared__ uint32_t cnt[kMaxTilesPerRow];
-    __shared__ uint32_t part[kBlock];
+    __shared__ uint32_t s_wave[kBlock / 64];
     __shared__ uint32_t wmax[kBlock / 64];
     const GridBwdRow r = rows[blockIdx.y];
     for (uint32_t t = threadIdx.x; t < r.n_tiles; t += kBlock) cnt[t] = 0;
@@ -314,19 +327,9 @@ __global__ __launch_bounds__(kBlock) void bin_kernel(uint32_t N, uint32_t ld, ui
     const uint32_t t_lo = min(r.n_tiles, threadIdx.x * per), t_hi = min(r.n_tiles, t_lo + per);
     uint32_t sum = 0;
     for (uint32_t t = t_lo; t < t_hi; ++t) sum += cnt[t];
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int i = 0; i < kBlock; ++i) {
-            const uint32_t v = part[i];
-            part[i] = acc;
-            acc += v;
-        }
-    }
-    __syncthreads();
+    uint32_t acc;
+    block_exclusive_scan(sum, s_wave, acc);
     const size_t region = region_of(blockIdx.y, blockIdx.x, n_qblocks);
-    uint32_t acc = part[threadIdx.x];
     for (uint32_t t = t_lo; t < t_hi; ++t) {
         const uint32_t c = cnt[t];
         const size_t at = (size_t)(r.count_base + t * r.reps + r.rep) * n_qblocks + blockIdx.x;
