@@ -31,7 +31,7 @@ class GridBackward {
   private:
     uint32_t queries_per_pass() const;
     uint64_t max_groups(uint32_t n) const;
-    void prepare(uint32_t N);
+    void prepare(hipStream_t s, uint32_t N);
 
     bool supported_ = false, uploaded_ = false;
     uint32_t F_ = 0, n_rows_ = 0, n_bins_ = 0, corners_per_query_ = 0, n_input_dims_ = 0;

@@ -149,6 +149,10 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     return total;
 }
 
+// meta: [0] groups, [1] max |dL/dy| bits being raised by the bin pass, [2]
+// tiles listed for the apply pass, [3] the pass's max, moved there from [1]
+// by the group scan, which leaves [1] zero for the next pass.  Cleared once,
+// when the buffer is allocated.
 __global__ void reset_meta_kernel(uint32_t* meta) {
     if (threadIdx.x < 4) meta[threadIdx.x] = 0;
 }
@@ -180,6 +184,7 @@ __global__ __launch_bounds__(kBlock) void scan_rows_kernel(uint32_t n_qblocks, c
 // Pass 2b (one block): group numbering (a group = up to chunk_recs records
 // of one tile).  meta[0] = groups.  Also lists, in order, the tiles the apply
 // pass has work for (several groups, or none when overwriting): meta[2] of them.
+// meta[3] = the max the bin pass raised in meta[1], which becomes 0 again.
 __device__ __forceinline__ uint32_t groups_of(uint32_t total, uint32_t chunk_recs) {
     return (total + chunk_recs - 1) / chunk_recs;
 }
@@ -223,6 +228,8 @@ __global__ __launch_bounds__(kBlock) void scan_bins_kernel(uint32_t n_bins, uint
     if (threadIdx.x == 0) {
         meta[0] = (uint32_t)(total >> 32);
         meta[2] = (uint32_t)total;
+        meta[3] = meta[1];
+        meta[1] = 0;
         group_start[n_bins] = (uint32_t)(total >> 32);
     }
     uint32_t ac = (uint32_t)(excl >> 32), aa = (uint32_t)excl;
@@ -268,7 +275,7 @@ __device__ __forceinline__ size_t region_of(uint32_t row, uint32_t qblock, uint3
 
 // Pass 1: records of (query block, row); run_len / run_addr[count row of
 // (tile, repetition)][query block] = the run's record count and its first
-// slot; meta[1] = max |dL/dy| bits.  grid (query blocks, rows).
+// slot; raises meta[1] (0 on entry) to max |dL/dy| bits.  grid (query blocks, rows).
 template <int F>
 __global__ __launch_bounds__(kBlock) void bin_kernel(uint32_t N, uint32_t ld, uint32_t n_qblocks, uint32_t in_stride,
                                                      const GridBwdRow* __restrict__ rows, const float* __restrict__ in,
@@ -432,7 +439,7 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(uint32_t n_bins, uint32_
     if (threadIdx.x == 0) s_pre[n_runs] = total;
     const uint32_t begin = (gi - group_start[b]) * kGroupRecs;
     const uint32_t end = min(total, begin + kGroupRecs);
-    const uint32_t mbits = meta[1];
+    const uint32_t mbits = meta[3];
     const bool finite = finite_bits(mbits);
     const int S = finite ? fixed_shift(mbits) : 0;
     for (uint32_t i = threadIdx.x; i < kTileFloats; i += kBlock) acc[i] = 0ull;
@@ -521,7 +528,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(const GridBwdBin* __restr
                                                        float* __restrict__ grads, uint32_t F, int overwrite) {
     constexpr uint32_t kItems = kTileFloats / kBlock;
     const uint32_t n_items = meta[2] * kItems;
-    const uint32_t mbits = meta[1];
+    const uint32_t mbits = meta[3];
     for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
         const uint32_t b = apply_list[item / kItems];
         const uint32_t c0 = group_start[b], c1 = group_start[b + 1];
@@ -652,7 +659,7 @@ uint32_t GridBackward::queries_per_pass() const {
                                         std::min(cap, runs_cap) / kQueriesPerBlock * kQueriesPerBlock);
 }
 
-void GridBackward::prepare(uint32_t N) {
+void GridBackward::prepare(hipStream_t s, uint32_t N) {
     const uint32_t n = std::min(N, queries_per_pass());
     const uint32_t nqb = div_round_up(n, kQueriesPerBlock);
     counts_.ensure(sizeof(uint32_t) * (size_t)count_rows_ * nqb);
@@ -660,7 +667,11 @@ void GridBackward::prepare(uint32_t N) {
     totals_.ensure(sizeof(uint32_t) * n_bins_);
     chunk_start_.ensure(sizeof(uint32_t) * (n_bins_ + 1));
     apply_list_.ensure(sizeof(uint32_t) * n_bins_);
-    meta_.ensure(sizeof(uint32_t) * 4);
+    if (!meta_.p) {
+        meta_.ensure(sizeof(uint32_t) * 4);
+        hipLaunchKernelGGL(reset_meta_kernel, dim3(1), dim3(64), 0, s, meta_.as<uint32_t>());
+        NLOD_LAUNCH_CHECK();
+    }
     const size_t rec_words = F_ + 2 <= 4 ? 4 : F_ + 2 <= 8 ? 8 : 16;
     // every (row, query block) has a region for its largest record count
     records_.ensure(sizeof(uint32_t) * (size_t)n_rows_ * nqb * kQueriesPerBlock * 8 * rec_words);
@@ -681,7 +692,7 @@ void GridBackward::run(hipStream_t s, const float* in, uint32_t N, const float* 
         uploaded_ = true;
     }
     const uint32_t pass = queries_per_pass();
-    prepare(N);
+    prepare(s, N);
     for (uint32_t q0 = 0; q0 < N; q0 += pass) {
         const uint32_t n = std::min(pass, N - q0);
         const uint32_t nqb = div_round_up(n, kQueriesPerBlock);
@@ -689,8 +700,6 @@ void GridBackward::run(hipStream_t s, const float* in, uint32_t N, const float* 
         const float* in_c = in + (size_t)q0 * n_input_dims_;
         // d_enc is SoA [K][N]: the pass's columns start at q0 within each row of stride N
         const float* denc_c = d_enc + q0;
-        hipLaunchKernelGGL(reset_meta_kernel, dim3(1), dim3(64), 0, s, meta_.as<uint32_t>());
-        NLOD_LAUNCH_CHECK();
         by_F(F_, [&](auto Fc) {
             constexpr int F = decltype(Fc)::value;
             hipLaunchKernelGGL(bin_kernel<F>, dim3(nqb, n_rows_), dim3(kBlock), 0, s, n, N, nqb, n_input_dims_,
