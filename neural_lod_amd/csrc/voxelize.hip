@@ -762,7 +762,8 @@ __global__ __launch_bounds__(kVoxThreads) void tile_scan_rows_kernel(uint32_t n_
 // the same block partition as the count pass).  Each block first scans the
 // tile totals itself for the first slot of each tile's list.  Block 0 also
 // stores those slots and the chunk numbering for the tile pass (chunk_start
-// has n_tiles + 1 entries; the last is the number of chunks), and moves the
+// has n_tiles + 1 entries; the last is the number of chunks; chunk_tile[c] is
+// the tile of chunk c), and moves the
 // count pass's direct-route triangles to misc[1], clearing misc[0] for the
 // next job.
 __global__ __launch_bounds__(kVoxThreads) void tile_scatter_kernel(uint64_t n_tri, TileGeo geo, uint32_t n_blocks,
@@ -771,6 +772,7 @@ __global__ __launch_bounds__(kVoxThreads) void tile_scatter_kernel(uint64_t n_tr
                                                                    const uint32_t* __restrict__ totals,
                                                                    uint32_t* __restrict__ start,
                                                                    uint32_t* __restrict__ chunk_start,
+                                                                   uint32_t* __restrict__ chunk_tile,
                                                                    uint32_t* __restrict__ misc,
                                                                    uint32_t* __restrict__ idx) {
     extern __shared__ uint32_t cursor[];
@@ -800,7 +802,9 @@ __global__ __launch_bounds__(kVoxThreads) void tile_scatter_kernel(uint64_t n_tr
         for (uint32_t t = t0; t < t1; ++t) {
             start[t] = cursor[t];
             chunk_start[t] = cex;
-            cex += chunks_of(t);
+            const uint32_t nc = chunks_of(t);
+            for (uint32_t k = 0; k < nc; ++k) chunk_tile[cex + k] = t;
+            cex += nc;
         }
         if (threadIdx.x == 0) {
             chunk_start[geo.n_tiles] = n_chunks;
@@ -863,6 +867,7 @@ __global__ __launch_bounds__(kVoxThreads) void tile_vox_kernel(VoxParams p, cons
                                                                TileGeo geo, const uint32_t* __restrict__ totals,
                                                                const uint32_t* __restrict__ start,
                                                                const uint32_t* __restrict__ chunk_start,
+                                                               const uint32_t* __restrict__ chunk_tile,
                                                                const uint32_t* __restrict__ idx,
                                                                const uint64_t* __restrict__ routes,
                                                                const uint32_t* __restrict__ misc,
@@ -873,13 +878,7 @@ __global__ __launch_bounds__(kVoxThreads) void tile_vox_kernel(VoxParams p, cons
     const uint32_t tile_words = geo.T * geo.wpp;
     const uint64_t GG = (uint64_t)p.G * p.G;
     for (uint32_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
-        uint32_t lo = 0, hi = geo.n_tiles;  // invariant: chunk_start[lo] <= c < chunk_start[hi]
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) / 2;
-            if (chunk_start[mid] <= c) lo = mid;
-            else hi = mid;
-        }
-        const uint32_t tile = lo;
+        const uint32_t tile = chunk_tile[c];
         const int ty = (int)(tile % geo.tiles_y), tz = (int)(tile / geo.tiles_y);
         const Rows clip{ty * (int)geo.T, min((ty + 1) * (int)geo.T, (int)p.G) - 1, tz * (int)geo.T,
                         min((tz + 1) * (int)geo.T, (int)p.G) - 1};
@@ -1199,13 +1198,20 @@ void launch_direct(hipStream_t stream, const VoxParams& p, const float* d_tris, 
     NLOD_LAUNCH_CHECK();
 }
 
+// Most chunks a job can have: one partial chunk per tile, plus the full ones.
+size_t max_chunks(const VoxParams& p, const TileGeo& geo) {
+    return (size_t)geo.n_tiles + (size_t)p.n_tri * kMaxTiles / geo.chunk + 1;
+}
+
 // Bytes of tiled-path scratch for a job of this geometry: misc | routes (u64
-// per triangle) | counts | totals | start | chunk_start (n_tiles + 1) | idx.
+// per triangle) | counts | totals | start | chunk_start (n_tiles + 1) |
+// chunk_tile (max_chunks) | idx.
 size_t tile_scratch_bytes(const VoxParams& p, const TileGeo& geo) {
     const uint32_t n_blocks = (uint32_t)((p.n_tri + geo.tris_per_block - 1) / geo.tris_per_block);
     const size_t n_counts = (size_t)geo.n_tiles * n_blocks;
     const size_t n_idx = (size_t)p.n_tri * kMaxTiles;
-    return (kMiscWords + 2 * (size_t)p.n_tri + n_counts + 3 * (size_t)geo.n_tiles + 1 + n_idx) * sizeof(uint32_t);
+    return (kMiscWords + 2 * (size_t)p.n_tri + n_counts + 3 * (size_t)geo.n_tiles + 1 + max_chunks(p, geo) + n_idx) *
+           sizeof(uint32_t);
 }
 
 // The tiled path's scratch comes from `workspace(bytes)`: the library's
@@ -1228,7 +1234,8 @@ void launch_voxelize(hipStream_t stream, const VoxParams& p, const TileGeo& geo,
     uint32_t* totals = counts + n_counts;
     uint32_t* start = totals + geo.n_tiles;
     uint32_t* chunk_start = start + geo.n_tiles;
-    uint32_t* idx = chunk_start + geo.n_tiles + 1;
+    uint32_t* chunk_tile = chunk_start + geo.n_tiles + 1;
+    uint32_t* idx = chunk_tile + max_chunks(p, geo);
     const size_t count_smem = (size_t)geo.n_tiles * 4 + kVoxThreads * 9 * sizeof(float);
     const size_t scatter_smem = (size_t)geo.n_tiles * 4;
     (void)occupancy_blocks(reinterpret_cast<const void*>(tile_count_kernel<SOLID>), kVoxThreads, count_smem);
@@ -1242,7 +1249,7 @@ void launch_voxelize(hipStream_t stream, const VoxParams& p, const TileGeo& geo,
                        totals);
     NLOD_LAUNCH_CHECK();
     hipLaunchKernelGGL(tile_scatter_kernel, dim3(n_blocks), dim3(kVoxThreads), scatter_smem, stream, p.n_tri, geo,
-                       n_blocks, routes, counts, totals, start, chunk_start, misc, idx);
+                       n_blocks, routes, counts, totals, start, chunk_start, chunk_tile, misc, idx);
     NLOD_LAUNCH_CHECK();
     size_t tile_smem = (size_t)geo.T * geo.wpp * sizeof(uint32_t) +
                        (kVoxThreads + 1 + kVoxThreads / 64) * sizeof(uint32_t);
@@ -1251,7 +1258,7 @@ void launch_voxelize(hipStream_t stream, const VoxParams& p, const TileGeo& geo,
         const int per_cu = occupancy_blocks(reinterpret_cast<const void*>(kernel), kVoxThreads, tile_smem);
         const uint32_t grid = (uint32_t)(cu_count() * per_cu);
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(kVoxThreads), tile_smem, stream, p, d_tris, geo, totals, start,
-                           chunk_start, idx, routes, misc, d_table);
+                           chunk_start, chunk_tile, idx, routes, misc, d_table);
         NLOD_LAUNCH_CHECK();
     };
     if (SOLID && geo.aligned) run_tiles(tile_vox_kernel<SOLID, true>);
